@@ -1,0 +1,283 @@
+/*
+ * pe.h -- C ABI of the MI355X Monte Carlo / RQMC pricing engine ("pe").
+ *
+ * The ABI replaces the simulation hot path behind the reference's facade:
+ *
+ *   PricingPlan::compile(&PricingRequest, ExecutionPolicy)
+ *       crates/pricing/src/engine/risk/pricing.rs:53-88
+ *   PricingPlan::evaluate(&self) -> MonteCarloPrice
+ *       crates/pricing/src/engine/risk/pricing.rs:86-88, 395-404
+ *   SimulationPlan::execute (the device boundary)
+ *       crates/pricing/src/engine/risk/valuation.rs:19-35
+ *
+ * Everything crossing the boundary is plain data: POD structs, pointers and
+ * sizes.  No exceptions cross it; every entry point returns a pe_status and
+ * pe_last_error() returns a thread-local message for the last failure.
+ * Dates are whole days counted from the valuation date (ACT/365F year fraction
+ * = days / 365, crates/pricing/src/core/date.rs:223-231).
+ *
+ * The oracle library (oracle/liboracle.so) accepts the same request structs
+ * through oracle_evaluate(); it is test infrastructure, not part of this ABI.
+ */
+#ifndef PE_H
+#define PE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PE_ABI_VERSION 1
+
+/* Error taxonomy mirrors MonteCarloError (crates/pricing/src/api/error.rs:256-300)
+ * at the granularity a C caller can act on. */
+typedef enum pe_status {
+    PE_OK = 0,
+    PE_ERR_INVALID_ARGUMENT = 1,   /* request validation failed at compile */
+    PE_ERR_UNSUPPORTED = 2,        /* product/model/engine combination not implemented */
+    PE_ERR_NONFINITE_STATE = 3,    /* a sampling unit produced a non-finite state */
+    PE_ERR_DEVICE = 4,             /* HIP runtime failure */
+    PE_ERR_ALLOCATION = 5,         /* host or device allocation failed */
+    PE_ERR_INSUFFICIENT_UNITS = 6, /* fewer than two independent units for a variance */
+    PE_ERR_SIZE_LIMIT = 7          /* request exceeds a documented engine limit */
+} pe_status;
+
+typedef enum pe_backend {
+    PE_BACKEND_GPU_FP64 = 1,  /* all-FP64 device mode: parity mode vs the oracle */
+    PE_BACKEND_GPU_MIXED = 2  /* FP32 path state, FP64 accumulators */
+} pe_backend;
+
+/* ---------------------------------------------------------------- market -- */
+
+/* LogLinearDiscountCurve (crates/pricing/src/market/curve.rs:39-212). */
+typedef struct pe_curve {
+    uint32_t count;
+    const double* times;
+    const double* discount_factors;
+} pe_curve;
+
+/* DividendQuote::fixed_cash_and_proportional (crates/pricing/src/market/dividend.rs). */
+typedef struct pe_dividend_quote {
+    int32_t ex_day;
+    double cash;
+    double proportional;
+} pe_dividend_quote;
+
+typedef struct pe_market {
+    double spot;
+    pe_curve discount_curve;
+    pe_curve dividend_curve;
+    uint32_t dividend_count;
+    const pe_dividend_quote* dividends;
+} pe_market;
+
+/* ----------------------------------------------------------------- model -- */
+
+typedef enum pe_model_kind {
+    PE_MODEL_BLACK_SCHOLES = 0,
+    PE_MODEL_BLACK76 = 1,
+    PE_MODEL_LOCAL_VOL = 2
+} pe_model_kind;
+
+/* LocalVarianceGrid (crates/pricing/src/market/local_variance.rs:128-135):
+ * values are row-major [time][log-forward-moneyness]. */
+typedef struct pe_local_variance_grid {
+    uint32_t time_count;
+    uint32_t x_count;
+    const double* time_nodes;
+    const double* x_nodes;
+    const double* values;
+    double floor;
+    double cap;
+} pe_local_variance_grid;
+
+typedef struct pe_model {
+    pe_model_kind kind;
+    double volatility;              /* BS / Black-76 */
+    pe_local_variance_grid lv;      /* local vol */
+} pe_model;
+
+/* --------------------------------------------------------------- product -- */
+
+typedef enum pe_product_kind {
+    PE_PRODUCT_EUROPEAN = 0,
+    PE_PRODUCT_DIGITAL = 1,
+    PE_PRODUCT_BARRIER = 2,
+    PE_PRODUCT_ASIAN = 3,
+    PE_PRODUCT_LOOKBACK = 4
+} pe_product_kind;
+
+typedef enum pe_side { PE_CALL = 0, PE_PUT = 1 } pe_side;
+
+/* ProductSpec (crates/pricing/src/product/spec.rs:14-140, 838-913).
+ * dates[]: Asian observation dates, Barrier monitoring dates or Lookback
+ * monitoring dates, ascending.  weights[]/fixings[] are per date for Asians
+ * (fixings[i] is NaN for an unknown observation). */
+typedef struct pe_product {
+    pe_product_kind kind;
+    pe_side side;
+    int32_t expiry_day;
+    int32_t payment_day;
+    double strike;
+    double notional;
+    /* Digital */
+    double payout;
+    int32_t payout_is_asset;
+    /* Barrier */
+    double barrier;
+    int32_t direction_down;     /* 0 = up, 1 = down */
+    int32_t knock_in;           /* 0 = knock-out, 1 = knock-in */
+    int32_t continuous;         /* 0 = discrete monitoring, 1 = continuous */
+    int32_t has_rebate;
+    double rebate;
+    int32_t historical_hit;     /* -1 = none, 0 = not hit, 1 = hit */
+    /* Lookback */
+    int32_t has_historical_extremum;
+    double historical_extremum;
+    /* schedules */
+    uint32_t date_count;
+    const int32_t* dates;
+    const double* weights;
+    const double* fixings;
+} pe_product;
+
+/* ---------------------------------------------------------------- engine -- */
+
+typedef enum pe_engine_kind {
+    PE_ENGINE_PSEUDO = 0, /* PseudoMcConfig (crates/pricing/src/engine/mc/config.rs) */
+    PE_ENGINE_RQMC = 1    /* RqmcConfig */
+} pe_engine_kind;
+
+typedef struct pe_engine {
+    pe_engine_kind kind;
+    uint64_t seed;              /* master seed / master scramble seed */
+    uint64_t units;             /* pseudo: independent units; RQMC: points per scramble */
+    uint32_t scrambles;         /* RQMC only */
+    int32_t antithetic;
+    int32_t brownian_bridge;
+} pe_engine;
+
+/* RiskRequest (crates/pricing/src/risk/request.rs). */
+typedef struct pe_risk {
+    int32_t delta;
+    int32_t vega;
+    int32_t gamma;              /* 0 = off */
+    int32_t gamma_relative;     /* 1: bump = spot * gamma_bump, 0: absolute */
+    double gamma_bump;
+    int32_t smoothing;          /* 1: PayoffSmoothing::CompactC2 */
+    double smoothing_half_width;
+} pe_risk;
+
+typedef struct pe_request {
+    pe_market market;
+    pe_model model;
+    pe_product product;
+    pe_engine engine;
+    pe_risk risk;
+} pe_request;
+
+/* ExecutionPolicy (crates/pricing/src/engine/mc/execution.rs:20-68). */
+typedef struct pe_exec_policy {
+    uint32_t worker_threads;
+    uint64_t reduction_block_size;   /* 0 selects the default 4096 */
+} pe_exec_policy;
+
+/* ---------------------------------------------------------------- result -- */
+
+/* Estimate (crates/pricing/src/engine/risk/report.rs:454-473). */
+typedef struct pe_estimate {
+    double value;
+    double standard_error;
+    double confidence_lower;
+    double confidence_upper;
+    uint64_t effective_units;
+} pe_estimate;
+
+/* Component order of the 18-wide pathwise vector
+ * (crates/pricing/src/engine/plan/simulation.rs:26-62). */
+enum {
+    PE_C_PRICE = 0, PE_C_DELTA = 1, PE_C_VEGA = 2, PE_C_GAMMA = 3,
+    PE_C_BUMP_DELTA = 4, PE_C_DELTA_DIFFERENCE = 5, PE_C_BUMP_VEGA = 6,
+    PE_C_VEGA_DIFFERENCE = 7, PE_C_BUMP_GAMMA = 8, PE_C_GAMMA_DIFFERENCE = 9,
+    PE_C_BARRIER_ENDPOINT_HIT = 10, PE_C_BARRIER_DIVIDEND_JUMP_HIT = 11,
+    PE_C_BARRIER_BRIDGE_HIT_WEIGHT = 12, PE_C_BARRIER_INTERVAL_COUNT = 13,
+    PE_C_BARRIER_FINITE_CORRECTION_COUNT = 14, PE_C_BARRIER_ZERO_VARIANCE_COUNT = 15,
+    PE_C_BARRIER_SURVIVAL_UNDERFLOW_COUNT = 16, PE_C_BARRIER_CERTAIN_SURVIVAL_COUNT = 17,
+    PE_COMPONENTS = 18
+};
+
+typedef struct pe_result {
+    /* per component, over independent units (pseudo) or scramble means (RQMC) */
+    pe_estimate estimates[PE_COMPONENTS];
+    double component_sum[PE_COMPONENTS];      /* Neumaier total */
+    double component_mean[PE_COMPONENTS];     /* centred-moment mean */
+    double component_m2[PE_COMPONENTS];       /* centred second moment */
+    double sampling_variance;
+    double estimator_variance;
+    uint64_t independent_units;
+    uint64_t evaluated_paths;
+    uint64_t path_steps;                      /* simulated (path, node) updates */
+    uint32_t time_steps;                      /* steps per path */
+    uint32_t sobol_dimension;                 /* RQMC effective dimension */
+    double device_ms;                         /* kernel time, HIP events */
+    uint64_t failing_unit;                    /* first failing sampling unit */
+} pe_result;
+
+/* ------------------------------------------------------------ entry points -- */
+
+typedef struct pe_plan pe_plan;
+
+/* Library bring-up; idempotent.  Selects HIP device `device`. */
+pe_status pe_init(int32_t device);
+const char* pe_last_error(void);
+int32_t pe_abi_version(void);
+
+/* Mirrors SimulationPlan::compile (crates/pricing/src/engine/compile/simulation.rs:78-640):
+ * host FP64 plan compilation, flattening to device tables, upload. */
+pe_status pe_plan_compile(const pe_request* request, const pe_exec_policy* policy,
+                          pe_plan** out);
+void pe_plan_free(pe_plan* plan);
+
+/* Mirrors PricingPlan::evaluate (crates/pricing/src/engine/risk/pricing.rs:86-88). */
+pe_status pe_plan_evaluate(const pe_plan* plan, pe_backend backend, pe_result* out);
+
+/* Evaluates several plans that share one simulated path set (same market,
+ * model, engine and simulation grid; e.g. a barrier and an Asian on the same
+ * local-vol paths) in one launch.  Each result equals what pe_plan_evaluate
+ * returns for that plan.  No reference counterpart: the reference compiles one
+ * plan per request and simulates each separately. */
+pe_status pe_plans_evaluate_shared(const pe_plan* const* plans, uint32_t count,
+                                   pe_backend backend, pe_result* out);
+
+/* LocalVarianceGrid::from_surface for a StandardSsvi surface with a ThetaPchip
+ * total-variance curve and a power-law phi, computed in FP64 on the device
+ * (crates/pricing/src/market/local_variance.rs:149-206, ssvi.rs:61-186,226-275,
+ * 340-475, pchip.rs:111-170).  values_out is row-major [time][x];
+ * repair_count_out receives the number of floor/cap repairs. */
+typedef struct pe_ssvi_surface {
+    uint32_t theta_count;
+    const double* theta_times;
+    const double* theta_values;
+    double theta_terminal_slope;
+    double rho;
+    double eta;
+    double gamma;
+} pe_ssvi_surface;
+
+pe_status pe_local_variance_from_ssvi(const pe_ssvi_surface* surface,
+                                      uint32_t time_count, const double* time_nodes,
+                                      uint32_t x_count, const double* x_nodes,
+                                      double floor, double cap,
+                                      double* values_out, uint32_t* repair_count_out);
+
+/* Kernel-time instrumentation of the last evaluate on this thread: average
+ * duration of the dominant kernel over its launches, in milliseconds. */
+pe_status pe_last_kernel_stats(double* avg_ms, uint32_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PE_H */

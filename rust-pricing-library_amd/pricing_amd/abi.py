@@ -1,0 +1,235 @@
+"""ctypes mirror of include/pe.h (the engine's C ABI)."""
+import ctypes as C
+
+PE_ABI_VERSION = 1
+
+# pe_status
+PE_OK = 0
+PE_ERR_INVALID_ARGUMENT = 1
+PE_ERR_UNSUPPORTED = 2
+PE_ERR_NONFINITE_STATE = 3
+PE_ERR_DEVICE = 4
+PE_ERR_ALLOCATION = 5
+PE_ERR_INSUFFICIENT_UNITS = 6
+PE_ERR_SIZE_LIMIT = 7
+
+# pe_backend
+PE_BACKEND_GPU_FP64 = 1
+PE_BACKEND_GPU_MIXED = 2
+
+# pe_model_kind
+PE_MODEL_BLACK_SCHOLES = 0
+PE_MODEL_BLACK76 = 1
+PE_MODEL_LOCAL_VOL = 2
+
+# pe_product_kind
+PE_PRODUCT_EUROPEAN = 0
+PE_PRODUCT_DIGITAL = 1
+PE_PRODUCT_BARRIER = 2
+PE_PRODUCT_ASIAN = 3
+PE_PRODUCT_LOOKBACK = 4
+
+PE_CALL = 0
+PE_PUT = 1
+
+PE_ENGINE_PSEUDO = 0
+PE_ENGINE_RQMC = 1
+
+COMPONENTS = 18
+COMPONENT_NAMES = (
+    "price", "delta", "vega", "gamma", "bump_delta", "delta_difference", "bump_vega",
+    "vega_difference", "bump_gamma", "gamma_difference", "barrier_endpoint_hit",
+    "barrier_dividend_jump_hit", "barrier_bridge_hit_weight", "barrier_interval_count",
+    "barrier_finite_correction_count", "barrier_zero_variance_count",
+    "barrier_survival_underflow_count", "barrier_certain_survival_count",
+)
+
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int32)
+
+
+class pe_curve(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("times", _DP), ("discount_factors", _DP)]
+
+
+class pe_dividend_quote(C.Structure):
+    _fields_ = [("ex_day", C.c_int32), ("cash", C.c_double), ("proportional", C.c_double)]
+
+
+class pe_market(C.Structure):
+    _fields_ = [
+        ("spot", C.c_double),
+        ("discount_curve", pe_curve),
+        ("dividend_curve", pe_curve),
+        ("dividend_count", C.c_uint32),
+        ("dividends", C.POINTER(pe_dividend_quote)),
+    ]
+
+
+class pe_local_variance_grid(C.Structure):
+    _fields_ = [
+        ("time_count", C.c_uint32),
+        ("x_count", C.c_uint32),
+        ("time_nodes", _DP),
+        ("x_nodes", _DP),
+        ("values", _DP),
+        ("floor", C.c_double),
+        ("cap", C.c_double),
+    ]
+
+
+class pe_model(C.Structure):
+    _fields_ = [("kind", C.c_int), ("volatility", C.c_double), ("lv", pe_local_variance_grid)]
+
+
+class pe_product(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int),
+        ("side", C.c_int),
+        ("expiry_day", C.c_int32),
+        ("payment_day", C.c_int32),
+        ("strike", C.c_double),
+        ("notional", C.c_double),
+        ("payout", C.c_double),
+        ("payout_is_asset", C.c_int32),
+        ("barrier", C.c_double),
+        ("direction_down", C.c_int32),
+        ("knock_in", C.c_int32),
+        ("continuous", C.c_int32),
+        ("has_rebate", C.c_int32),
+        ("rebate", C.c_double),
+        ("historical_hit", C.c_int32),
+        ("has_historical_extremum", C.c_int32),
+        ("historical_extremum", C.c_double),
+        ("date_count", C.c_uint32),
+        ("dates", _IP),
+        ("weights", _DP),
+        ("fixings", _DP),
+    ]
+
+
+class pe_engine(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int),
+        ("seed", C.c_uint64),
+        ("units", C.c_uint64),
+        ("scrambles", C.c_uint32),
+        ("antithetic", C.c_int32),
+        ("brownian_bridge", C.c_int32),
+    ]
+
+
+class pe_risk(C.Structure):
+    _fields_ = [
+        ("delta", C.c_int32),
+        ("vega", C.c_int32),
+        ("gamma", C.c_int32),
+        ("gamma_relative", C.c_int32),
+        ("gamma_bump", C.c_double),
+        ("smoothing", C.c_int32),
+        ("smoothing_half_width", C.c_double),
+    ]
+
+
+class pe_request(C.Structure):
+    _fields_ = [
+        ("market", pe_market),
+        ("model", pe_model),
+        ("product", pe_product),
+        ("engine", pe_engine),
+        ("risk", pe_risk),
+    ]
+
+
+class pe_exec_policy(C.Structure):
+    _fields_ = [("worker_threads", C.c_uint32), ("reduction_block_size", C.c_uint64)]
+
+
+class pe_estimate(C.Structure):
+    _fields_ = [
+        ("value", C.c_double),
+        ("standard_error", C.c_double),
+        ("confidence_lower", C.c_double),
+        ("confidence_upper", C.c_double),
+        ("effective_units", C.c_uint64),
+    ]
+
+
+class pe_result(C.Structure):
+    _fields_ = [
+        ("estimates", pe_estimate * COMPONENTS),
+        ("component_sum", C.c_double * COMPONENTS),
+        ("component_mean", C.c_double * COMPONENTS),
+        ("component_m2", C.c_double * COMPONENTS),
+        ("sampling_variance", C.c_double),
+        ("estimator_variance", C.c_double),
+        ("independent_units", C.c_uint64),
+        ("evaluated_paths", C.c_uint64),
+        ("path_steps", C.c_uint64),
+        ("time_steps", C.c_uint32),
+        ("sobol_dimension", C.c_uint32),
+        ("device_ms", C.c_double),
+        ("failing_unit", C.c_uint64),
+    ]
+
+
+class pe_ssvi_surface(C.Structure):
+    _fields_ = [
+        ("theta_count", C.c_uint32),
+        ("theta_times", _DP),
+        ("theta_values", _DP),
+        ("theta_terminal_slope", C.c_double),
+        ("rho", C.c_double),
+        ("eta", C.c_double),
+        ("gamma", C.c_double),
+    ]
+
+
+# Every symbol include/pe.h declares, with (restype, argtypes).
+PE_SYMBOLS = {
+    "pe_init": (C.c_int, [C.c_int32]),
+    "pe_last_error": (C.c_char_p, []),
+    "pe_abi_version": (C.c_int32, []),
+    "pe_plan_compile": (C.c_int, [C.POINTER(pe_request), C.POINTER(pe_exec_policy), C.POINTER(C.c_void_p)]),
+    "pe_plan_free": (None, [C.c_void_p]),
+    "pe_plan_evaluate": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(pe_result)]),
+    "pe_plans_evaluate_shared": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.c_int, C.POINTER(pe_result)]),
+    "pe_local_variance_from_ssvi": (
+        C.c_int,
+        [C.POINTER(pe_ssvi_surface), C.c_uint32, _DP, C.c_uint32, _DP, C.c_double, C.c_double, _DP,
+         C.POINTER(C.c_uint32)],
+    ),
+    "pe_last_kernel_stats": (C.c_int, [_DP, C.POINTER(C.c_uint32)]),
+}
+
+
+def bind(lib, symbols=PE_SYMBOLS):
+    for name, (restype, argtypes) in symbols.items():
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+    return lib
+
+
+def estimate_dict(e):
+    return {
+        "value": e.value,
+        "standard_error": e.standard_error,
+        "confidence_lower": e.confidence_lower,
+        "confidence_upper": e.confidence_upper,
+        "effective_sampling_units": e.effective_units,
+    }
+
+
+def result_dict(r):
+    out = {name: estimate_dict(r.estimates[i]) for i, name in enumerate(COMPONENT_NAMES)}
+    out["sampling_variance"] = r.sampling_variance
+    out["estimator_variance"] = r.estimator_variance
+    out["independent_sampling_units"] = r.independent_units
+    out["evaluated_paths"] = r.evaluated_paths
+    out["path_steps"] = r.path_steps
+    out["time_steps"] = r.time_steps
+    out["sobol_dimension"] = r.sobol_dimension
+    out["device_ms"] = r.device_ms
+    out["component_sum"] = list(r.component_sum)
+    return out

@@ -1,0 +1,245 @@
+"""PricingRequest mirror: builds the C-ABI request from the reference's
+schema-v3 request documents.
+
+Field names and meaning follow ``schemas/v3/pricing_request.schema.json`` and
+``PricingRequest`` (crates/pricing/src/api/request.rs:11-300).  Dates become
+whole days from the valuation date (ACT/365F, crates/pricing/src/core/date.rs:223-231).
+Only the single-asset products and models of the simulation hot path are
+accepted; anything else raises ``UnsupportedRequest``.
+"""
+import ctypes as C
+import datetime as _dt
+import math
+
+from . import abi
+
+
+class UnsupportedRequest(ValueError):
+    pass
+
+
+def _days(valuation, date):
+    return (_dt.date.fromisoformat(date) - valuation).days
+
+
+def _darr(values):
+    arr = (C.c_double * len(values))(*values)
+    return arr
+
+
+def _iarr(values):
+    return (C.c_int32 * len(values))(*values)
+
+
+class Request:
+    """A C-ABI request plus the buffers it points into (kept alive here)."""
+
+    def __init__(self):
+        self.c = abi.pe_request()
+        self._keep = []
+        self.doc = None
+
+    def _ptr_d(self, values):
+        arr = _darr(values)
+        self._keep.append(arr)
+        return C.cast(arr, C.POINTER(C.c_double))
+
+    def _ptr_i(self, values):
+        arr = _iarr(values)
+        self._keep.append(arr)
+        return C.cast(arr, C.POINTER(C.c_int32))
+
+    def pointer(self):
+        return C.pointer(self.c)
+
+    # -------------------------------------------------------------- builders
+    def set_market(self, spot, discount_times, discount_factors, dividend_times, dividend_factors,
+                   dividends=()):
+        m = self.c.market
+        m.spot = float(spot)
+        m.discount_curve.count = len(discount_times)
+        m.discount_curve.times = self._ptr_d(discount_times)
+        m.discount_curve.discount_factors = self._ptr_d(discount_factors)
+        m.dividend_curve.count = len(dividend_times)
+        m.dividend_curve.times = self._ptr_d(dividend_times)
+        m.dividend_curve.discount_factors = self._ptr_d(dividend_factors)
+        if dividends:
+            arr = (abi.pe_dividend_quote * len(dividends))()
+            for i, (day, cash, prop) in enumerate(dividends):
+                arr[i].ex_day = int(day)
+                arr[i].cash = float(cash)
+                arr[i].proportional = float(prop)
+            self._keep.append(arr)
+            m.dividend_count = len(dividends)
+            m.dividends = C.cast(arr, C.POINTER(abi.pe_dividend_quote))
+        return self
+
+    def set_black_scholes(self, volatility, black76=False):
+        self.c.model.kind = abi.PE_MODEL_BLACK76 if black76 else abi.PE_MODEL_BLACK_SCHOLES
+        self.c.model.volatility = float(volatility)
+        return self
+
+    def set_local_vol(self, time_nodes, x_nodes, values, floor, cap):
+        self.c.model.kind = abi.PE_MODEL_LOCAL_VOL
+        g = self.c.model.lv
+        g.time_count = len(time_nodes)
+        g.x_count = len(x_nodes)
+        g.time_nodes = self._ptr_d(time_nodes)
+        g.x_nodes = self._ptr_d(x_nodes)
+        g.values = self._ptr_d(values)
+        g.floor = float(floor)
+        g.cap = float(cap)
+        return self
+
+    def set_product(self, kind, side, expiry_day, strike, notional=1.0, payment_day=None, **kw):
+        p = self.c.product
+        p.kind = kind
+        p.side = side
+        p.expiry_day = int(expiry_day)
+        p.payment_day = int(expiry_day if payment_day is None else payment_day)
+        p.strike = float(strike)
+        p.notional = float(notional)
+        p.payout = float(kw.get("payout", 0.0))
+        p.payout_is_asset = int(kw.get("payout_is_asset", 0))
+        p.barrier = float(kw.get("barrier", 0.0))
+        p.direction_down = int(kw.get("direction_down", 0))
+        p.knock_in = int(kw.get("knock_in", 0))
+        p.continuous = int(kw.get("continuous", 0))
+        rebate = kw.get("rebate")
+        p.has_rebate = 0 if rebate is None else 1
+        p.rebate = 0.0 if rebate is None else float(rebate)
+        hh = kw.get("historical_hit")
+        p.historical_hit = -1 if hh is None else int(bool(hh))
+        he = kw.get("historical_extremum")
+        p.has_historical_extremum = 0 if he is None else 1
+        p.historical_extremum = 0.0 if he is None else float(he)
+        dates = kw.get("dates", ())
+        p.date_count = len(dates)
+        p.dates = self._ptr_i(dates) if dates else None
+        weights = kw.get("weights")
+        p.weights = self._ptr_d(weights) if weights else None
+        fixings = kw.get("fixings")
+        p.fixings = self._ptr_d([math.nan if f is None else f for f in fixings]) if fixings else None
+        return self
+
+    def set_pseudo(self, seed, units, antithetic=False, brownian_bridge=False):
+        e = self.c.engine
+        e.kind = abi.PE_ENGINE_PSEUDO
+        e.seed = int(seed)
+        e.units = int(units)
+        e.scrambles = 0
+        e.antithetic = int(bool(antithetic))
+        e.brownian_bridge = int(bool(brownian_bridge))
+        return self
+
+    def set_rqmc(self, seed, points, scrambles, antithetic=False, brownian_bridge=False):
+        e = self.c.engine
+        e.kind = abi.PE_ENGINE_RQMC
+        e.seed = int(seed)
+        e.units = int(points)
+        e.scrambles = int(scrambles)
+        e.antithetic = int(bool(antithetic))
+        e.brownian_bridge = int(bool(brownian_bridge))
+        return self
+
+    def set_risk(self, delta=False, vega=False, gamma_bump=None, gamma_relative=True,
+                 smoothing_half_width=None):
+        r = self.c.risk
+        r.delta = int(bool(delta))
+        r.vega = int(bool(vega))
+        r.gamma = 0 if gamma_bump is None else 1
+        r.gamma_relative = int(bool(gamma_relative))
+        r.gamma_bump = 0.0 if gamma_bump is None else float(gamma_bump)
+        r.smoothing = 0 if smoothing_half_width is None else 1
+        r.smoothing_half_width = 0.0 if smoothing_half_width is None else float(smoothing_half_width)
+        return self
+
+    # ------------------------------------------------------- schema-v3 reader
+    @classmethod
+    def from_v3(cls, doc):
+        """Builds a request from a schema-v3 ``pricing_request`` document."""
+        self = cls()
+        self.doc = doc
+        valuation = _dt.date.fromisoformat(doc["valuation_date"])
+        mk = doc["market"]
+        if mk.get("type") != "equity":
+            raise UnsupportedRequest("only equity markets are supported")
+        dividends = []
+        for d in mk.get("discrete_dividends", []) or []:
+            raise UnsupportedRequest("discrete dividends in v3 documents are not supported yet")
+        self.set_market(mk["spot"], mk["discount_curve"]["times"], mk["discount_curve"]["discount_factors"],
+                        mk["dividend_curve"]["times"], mk["dividend_curve"]["discount_factors"], dividends)
+        model = doc["model"]
+        if model["type"] == "black_scholes":
+            self.set_black_scholes(model["volatility"])
+        elif model["type"] == "black_76":
+            self.set_black_scholes(model["volatility"], black76=True)
+        elif model["type"] == "local_volatility":
+            g = model["local_variance_grid"]
+            self.set_local_vol(g["time_nodes"], g["log_forward_moneyness_nodes"], g["values"], g["floor"], g["cap"])
+        else:
+            raise UnsupportedRequest(f"model {model['type']}")
+        pr = doc["product"]
+        side = abi.PE_CALL if pr.get("side", {}).get("type", "call") == "call" else abi.PE_PUT
+        kind = pr["type"]
+        notional = pr.get("notional", 1.0)
+        if kind == "european_vanilla":
+            self.set_product(abi.PE_PRODUCT_EUROPEAN, side, _days(valuation, pr["expiry"]), pr["strike"], notional,
+                             _days(valuation, pr.get("payment_date", pr["expiry"])))
+        elif kind == "digital":
+            self.set_product(abi.PE_PRODUCT_DIGITAL, side, _days(valuation, pr["expiry"]), pr["strike"], 1.0,
+                             _days(valuation, pr.get("payment_date", pr["expiry"])), payout=pr["payout"],
+                             payout_is_asset=int(pr["payout_kind"]["type"] == "asset"))
+        elif kind == "barrier":
+            self.set_product(
+                abi.PE_PRODUCT_BARRIER, side, _days(valuation, pr["expiry"]), pr["strike"], notional,
+                _days(valuation, pr.get("payment_date", pr["expiry"])), barrier=pr["barrier"],
+                direction_down=int(pr["direction"]["type"] == "down"),
+                knock_in=int(pr["style"]["type"] == "knock_in"),
+                continuous=int(pr["monitoring"]["type"] == "continuous"),
+                rebate=pr.get("rebate"), historical_hit=pr.get("historical_hit"),
+                dates=[_days(valuation, d) for d in pr["monitoring_dates"]])
+        elif kind == "arithmetic_asian":
+            obs = pr["observations"]
+            last = max(_dt.date.fromisoformat(o["date"]) for o in obs)
+            dates = [_days(valuation, o["date"]) for o in obs]
+            weights = [o["weight"] for o in obs]
+            fixings = [o["value"]["fixing"] if o["value"]["type"] == "known" else None for o in obs]
+            self.set_product(abi.PE_PRODUCT_ASIAN, side, (last - valuation).days, pr["strike"], notional,
+                             _days(valuation, pr.get("payment_date", last.isoformat())), dates=dates,
+                             weights=weights, fixings=fixings)
+        elif kind == "fixed_lookback":
+            dates = [_days(valuation, d) for d in pr["monitoring_dates"]]
+            self.set_product(abi.PE_PRODUCT_LOOKBACK, side, max(dates), pr["strike"], notional,
+                             _days(valuation, pr.get("payment_date", pr["monitoring_dates"][-1])), dates=dates,
+                             historical_extremum=pr.get("historical_extremum"))
+        else:
+            raise UnsupportedRequest(f"product {kind}")
+        eng = doc["engine"]
+        vr = eng.get("variance_reduction", {})
+        if eng["type"] == "pseudo_monte_carlo":
+            self.set_pseudo(eng["master_seed"], eng["independent_sampling_units"], vr.get("antithetic", False),
+                            vr.get("brownian_bridge", False))
+        elif eng["type"] == "randomized_quasi_monte_carlo":
+            self.set_rqmc(eng["master_scramble_seed"], eng["points_per_scramble"], eng["scramble_count"],
+                          vr.get("antithetic", False), vr.get("brownian_bridge", False))
+        else:
+            raise UnsupportedRequest(f"engine {eng['type']}")
+        risk = doc.get("risk", {})
+        gamma = risk.get("gamma")
+        gamma_bump = None
+        gamma_relative = True
+        if gamma:
+            gamma_bump = gamma["bump"]["value"]
+            gamma_relative = gamma["bump"]["type"] == "relative"
+        smoothing = risk.get("payoff_smoothing")
+        self.set_risk(risk.get("delta", False), risk.get("vega", False), gamma_bump, gamma_relative,
+                      smoothing["half_width"] if smoothing else None)
+        return self
+
+
+def exec_policy(worker_threads=1, reduction_block_size=0):
+    p = abi.pe_exec_policy()
+    p.worker_threads = int(worker_threads)
+    p.reduction_block_size = int(reduction_block_size)
+    return p

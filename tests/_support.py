@@ -1,0 +1,88 @@
+"""Shared loaders for the tests: the oracle (checker) and the engine (product)."""
+import ctypes as C
+import json
+import os
+import subprocess
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+_oracle = None
+
+
+def directions_path():
+    from pricing_amd import sobol_directions
+
+    path = sobol_directions.default_path()
+    if not os.path.exists(path):
+        sobol_directions.write(path)
+    return path
+
+
+def oracle():
+    """The CPU FP64 oracle (test infrastructure), built on first use if absent."""
+    global _oracle
+    if _oracle is not None:
+        return _oracle
+    if not os.path.exists(ORACLE_SO):
+        subprocess.check_call(["make", "-s", "-C", ORACLE_DIR])
+    from pricing_amd import abi
+
+    lib = C.CDLL(ORACLE_SO)
+    lib.oracle_init.argtypes = [C.c_char_p]
+    lib.oracle_last_error.restype = C.c_char_p
+    lib.oracle_philox.argtypes = [C.c_uint64, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    lib.oracle_philox_word.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32]
+    lib.oracle_philox_word.restype = C.c_uint32
+    lib.oracle_open_unit_interval.argtypes = [C.c_uint32]
+    lib.oracle_open_unit_interval.restype = C.c_double
+    lib.oracle_as241.argtypes = [C.c_double, C.POINTER(C.c_double)]
+    lib.oracle_sobol_word.argtypes = [C.c_uint32, C.c_uint32]
+    lib.oracle_sobol_word.restype = C.c_uint32
+    lib.oracle_rqmc_words.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32),
+                                      C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    D = C.POINTER(C.c_double)
+    U = C.POINTER(C.c_uint32)
+    lib.oracle_bridge_apply.argtypes = [D, C.c_uint32, D, D, U, U, D, D, D]
+    lib.oracle_bridge_reverse.argtypes = [D, C.c_uint32, D, D]
+    lib.oracle_neumaier.argtypes = [D, C.c_uint32]
+    lib.oracle_neumaier.restype = C.c_double
+    lib.oracle_two_pass.argtypes = [D, C.c_uint32, D]
+    lib.oracle_block_reduce.argtypes = [D, C.c_uint64, C.c_uint64, C.c_uint32, D]
+    lib.oracle_lv_time_grid.argtypes = [D, C.c_uint32, C.c_double, D, C.c_uint32, U]
+    lib.oracle_local_variance_from_ssvi.argtypes = [C.POINTER(abi.pe_ssvi_surface), C.c_uint32, D, C.c_uint32, D,
+                                                    C.c_double, C.c_double, D, U]
+    lib.oracle_ssvi_derivatives.argtypes = [C.POINTER(abi.pe_ssvi_surface), C.c_double, C.c_double, D]
+    lib.oracle_sinh_nodes.argtypes = [C.c_double, C.c_double, C.c_uint32, C.c_uint32, C.c_double, C.c_double, D]
+    lib.oracle_evaluate.argtypes = [C.POINTER(abi.pe_request), C.POINTER(abi.pe_exec_policy),
+                                    C.POINTER(abi.pe_result)]
+    lib.oracle_evaluate_shared.argtypes = [C.POINTER(C.POINTER(abi.pe_request)), C.c_uint32,
+                                           C.POINTER(abi.pe_exec_policy), C.POINTER(abi.pe_result)]
+    lib.oracle_hardware_threads.restype = C.c_uint
+    if lib.oracle_init(directions_path().encode()) != 0:
+        raise RuntimeError(lib.oracle_last_error().decode())
+    _oracle = lib
+    return lib
+
+
+def oracle_evaluate(req, workers=4, block=0):
+    from pricing_amd import abi, request
+
+    lib = oracle()
+    res = abi.pe_result()
+    pol = request.exec_policy(workers, block)
+    rc = lib.oracle_evaluate(req.pointer(), C.byref(pol), C.byref(res))
+    if rc != 0:
+        raise RuntimeError(f"oracle_evaluate failed ({rc}): {lib.oracle_last_error().decode()}")
+    return res
+
+
+def golden(name="reference_vectors.json"):
+    with open(os.path.join(GOLDEN, name)) as f:
+        return json.load(f)
+
+
+def darr(values):
+    return (C.c_double * len(values))(*values)

@@ -1,0 +1,62 @@
+"""Extracts golden vectors from the reference's own fixture files.
+
+Run in a container where the reference is mounted (default /root/reference);
+writes tests/golden/reference_vectors.json.  The output holds data only:
+request documents, expected estimates and Decimal reference values copied
+from
+
+  fixtures/replay/{european_bs,local_volatility,path_dependence}-windows-x86_64.json
+  fixtures/local-vol/reference-cases-v0.1.json  (standard_ssvi, constant_vol_dupire)
+
+The tests read the JSON this script writes; nothing at test time reads the
+reference tree.
+"""
+import json
+import os
+import sys
+
+ROOT = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def replay(name):
+    path = os.path.join(ROOT, "fixtures", "replay", name)
+    with open(path) as f:
+        doc = json.load(f)
+    cases = []
+    for case in doc["cases"]:
+        result = case["result"]
+        risks = {}
+        for key, value in (result.get("risks") or {}).items():
+            if isinstance(value, dict) and "raw" in value:
+                risks[key] = {"value": value["raw"]["value"], "standard_error": value["raw"]["standard_error"]}
+        cases.append({
+            "name": case["name"],
+            "source": f"fixtures/replay/{name}",
+            "request": case["request"],
+            "reduction_block_size": case["plan"]["reduction_block_size"],
+            "worker_threads": case["plan"]["worker_threads"],
+            "value": result["value"],
+            "risks": risks,
+            "sampling_variance_bits": case["execution"]["sampling_variance_bits"],
+            "evaluated_paths": case["execution"]["evaluated_paths"],
+        })
+    return cases
+
+
+def main():
+    out = {"replay": []}
+    for name in ("european_bs-windows-x86_64.json", "local_volatility-windows-x86_64.json",
+                 "path_dependence-windows-x86_64.json"):
+        out["replay"].extend(replay(name))
+    with open(os.path.join(ROOT, "fixtures", "local-vol", "reference-cases-v0.1.json")) as f:
+        lv = json.load(f)
+    out["standard_ssvi"] = [c for c in lv["standard_ssvi"] if "eta" in c["inputs"]]
+    out["constant_vol_dupire"] = lv["constant_vol_dupire"]
+    with open(os.path.join(HERE, "reference_vectors.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+        f.write("\n")
+
+
+if __name__ == "__main__":
+    main()

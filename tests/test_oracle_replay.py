@@ -1,0 +1,64 @@
+"""Pins the oracle end to end on the reference's frozen replay fixtures
+(fixtures/replay/*-windows-x86_64.json, extracted into tests/golden)."""
+import math
+import struct
+
+import pytest
+
+from _support import golden, oracle_evaluate
+from pricing_amd import request
+
+UNSUPPORTED = {
+    # local-vol Delta/Gamma/VegaKT risk and continuous barriers are not yet in
+    # the oracle; the price-only and discrete cases pin the same paths.
+    "pseudo_mc_delta_gamma_vega_vegakt",
+    "rqmc_delta_gamma_vega_vegakt",
+    "continuous_barrier_smoothed_full_risk",
+}
+
+CASES = [c for c in golden()["replay"]]
+
+
+def bits_to_float(bits):
+    return struct.unpack(">d", bytes.fromhex(bits))[0]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
+def test_replay_case(case):
+    if case["name"] in UNSUPPORTED:
+        pytest.skip("risk/continuous-barrier branch not restated yet")
+    req = request.Request.from_v3(case["request"])
+    res = oracle_evaluate(req, workers=4, block=case["reduction_block_size"])
+    exp = case["value"]
+    price = res.estimates[0]
+    # Same platform arithmetic except libm ulps: <= 1e-12 relative.
+    assert abs(price.value - exp["value"]) <= 1e-12 * abs(exp["value"])
+    assert abs(price.standard_error - exp["standard_error"]) <= 1e-10 * exp["standard_error"]
+    assert abs(res.sampling_variance - bits_to_float(case["sampling_variance_bits"])) <= 1e-10 * abs(
+        res.sampling_variance)
+    assert res.evaluated_paths == int(case["evaluated_paths"])
+    for key, index in (("delta", 1), ("vega", 2), ("gamma", 3)):
+        if key in case["risks"]:
+            e = case["risks"][key]
+            got = res.estimates[index]
+            assert abs(got.value - e["value"]) <= 1e-12 * max(1e-300, abs(e["value"])), key
+            assert abs(got.standard_error - e["standard_error"]) <= 1e-10 * e["standard_error"], key
+
+
+def test_replay_is_bit_identical_across_worker_counts():
+    case = next(c for c in CASES if c["name"] == "rqmc_price_only")
+    req = request.Request.from_v3(case["request"])
+    a = oracle_evaluate(req, workers=1, block=256)
+    b = oracle_evaluate(req, workers=8, block=256)
+    assert a.estimates[0].value == b.estimates[0].value
+    assert a.estimates[0].standard_error == b.estimates[0].standard_error
+
+
+def test_plausible_mistake_flips_a_replay_case():
+    # A flipped antithetic flag (a dropped term) must move the result well
+    # outside the 1e-12 replay band, so the replay check is discriminating.
+    case = next(c for c in CASES if c["name"] == "pseudo_mc_price_only")
+    req = request.Request.from_v3(case["request"])
+    req.c.engine.antithetic = 0
+    res = oracle_evaluate(req, workers=4, block=256)
+    assert abs(res.estimates[0].value - case["value"]["value"]) > 1e-6
