@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Headline benchmark: path-steps/s of the mixed-precision engine on the
+local-vol Log-Euler path-dependent workload, next to the all-FP64 device mode
+and the CPU oracle.
+
+Workload (one "step" = one evaluation of the whole job):
+  2^16 RQMC points x 16 scrambles = 2^20 paths, Sobol + LMS scrambles with
+  Brownian-bridge ordering, Log-Euler local vol over the union of a 32 x 64
+  Dupire grid (eSSVI-type StandardSsvi, theta(t) = 0.04 t, rho = -0.6,
+  eta = 1.0, gamma = 0.4; built in FP64 on the GPU) and 252 daily fixings;
+  products: up-and-out call K=100, B=130 with daily monitoring and an
+  arithmetic Asian call K=100, priced on the same paths in one launch.
+
+Prints one JSON line.  --gpus above 1 is refused: this build runs one GPU.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "rust-pricing-library_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+from pricing_amd import abi, engine, request, workloads  # noqa: E402
+
+METRIC = "path-steps/s at matched RQMC std-error vs FP64 oracle; NN-LSM vs poly-LSM time"
+FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
+FP64_PEAK_TFLOPS = 78.6    # MI355X published FP64 vector peak
+# Frozen work model (flops per path-step of the local-vol path-dependent set):
+# normal draw 34 + bridge 7 + LV Log-Euler step 24 + barrier monitor 4 +
+# Asian accumulate 2 + observation 2.
+FLOPS_PER_PATH_STEP = 73
+
+
+def build_plans(points, scrambles):
+    expiry = workloads.weekday_dates(252)[-1] / 365.0
+    grid = workloads.lv_dupire_grid(expiry, build=engine.local_variance_from_ssvi)
+    reqs = workloads.lv_path_dependent(grid, points=points, scrambles=scrambles, rate=0.05)
+    plans = [engine.PricingPlan.compile(r) for r in reqs]
+    return reqs, plans, grid
+
+
+def time_mode(plans, backend, steps, warmup):
+    for _ in range(warmup):
+        engine.evaluate_shared(plans, backend)
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res = engine.evaluate_shared(plans, backend)
+        kernel_ms.append(engine.last_kernel_ms()[0])
+    wall = time.perf_counter() - t0
+    return res, wall, kernel_ms
+
+
+def cpu_baseline(reqs, target_seconds):
+    """The CPU FP64 oracle on all host cores on a bounded sample of the same
+    workload (one scramble, a reduced point count)."""
+    from _support import oracle
+
+    lib = oracle()
+    cores = int(lib.oracle_hardware_threads())
+    points = 1 << 10
+    while True:
+        sample = []
+        for r in reqs:
+            q = request.Request()
+            q.c = abi.pe_request.from_buffer_copy(r.c)
+            q._keep = r._keep
+            q.c.engine.units = points
+            q.c.engine.scrambles = 2
+            sample.append(q)
+        arr = (C.POINTER(abi.pe_request) * len(sample))(*[q.pointer() for q in sample])
+        out = (abi.pe_result * len(sample))()
+        pol = request.exec_policy(cores, 64)
+        t0 = time.perf_counter()
+        rc = lib.oracle_evaluate_shared(arr, len(sample), C.byref(pol), out)
+        dt = time.perf_counter() - t0
+        if rc != 0:
+            raise RuntimeError(lib.oracle_last_error().decode())
+        if dt >= target_seconds or points >= 1 << 16:
+            break
+        points = min(1 << 16, points * max(2, int(target_seconds / max(dt, 1e-3))))
+        points = 1 << (points.bit_length() - 1)
+    path_steps = out[0].path_steps
+    return {"value": path_steps / dt, "unit": "path-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{points} points x 2 scrambles x {out[0].time_steps} steps ({path_steps} path-steps), "
+                      f"barrier + Asian on shared paths, FP64 oracle, {dt:.2f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--points", type=int, default=1 << 16)
+    ap.add_argument("--scrambles", type=int, default=16)
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build", file=sys.stderr)
+        return 2
+
+    engine.init(0)
+    reqs, plans, grid = build_plans(args.points, args.scrambles)
+    res, wall, kms = time_mode(plans, "mixed", args.steps, args.warmup)
+    path_steps = res[0].path_steps
+    ms_per_step = 1e3 * wall / args.steps
+    value = path_steps / (wall / args.steps)
+    avg_kernel_ms = sum(kms) / len(kms)
+    achieved = path_steps * FLOPS_PER_PATH_STEP / (avg_kernel_ms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "path-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "local-vol Log-Euler path-dependent set: 2^16 RQMC points x 16 scrambles (2^20 paths), "
+                        f"{res[0].time_steps} steps (252 daily fixings + 32x64 Dupire grid nodes), Sobol+LMS, "
+                        "Brownian bridge, up-and-out call K=100 B=130 + arithmetic Asian call K=100",
+            "paths": res[0].evaluated_paths,
+            "time_steps": res[0].time_steps,
+            "path_steps_per_step": path_steps,
+            "precision": "mixed: FP32 inverse normal, exp/sqrt, variance lookup; FP64 log-state, bridge, "
+                         "payoff accumulators and statistics",
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP32_PEAK_TFLOPS,
+            "traffic": None,
+            "kernel": "lv_kernel<float>",
+            "avg_kernel_ms": avg_kernel_ms,
+            "flops_per_path_step": FLOPS_PER_PATH_STEP,
+        },
+        "results": {
+            "barrier": {"price": res[0].estimates[0].value, "se": res[0].estimates[0].standard_error},
+            "asian": {"price": res[1].estimates[0].value, "se": res[1].estimates[0].standard_error},
+            "dupire_repairs": grid[5],
+        },
+    }
+    if not args.no_fp64:
+        f64, wall64, kms64 = time_mode(plans, "fp64", max(2, args.steps // 2), 1)
+        v64 = path_steps / (wall64 / max(2, args.steps // 2))
+        out["fp64_mode"] = {
+            "value": v64,
+            "ms_per_step": 1e3 * wall64 / max(2, args.steps // 2),
+            "avg_kernel_ms": sum(kms64) / len(kms64),
+            "mixed_over_fp64": value / v64,
+            "barrier": {"price": f64[0].estimates[0].value, "se": f64[0].estimates[0].standard_error},
+            "asian": {"price": f64[1].estimates[0].value, "se": f64[1].estimates[0].standard_error},
+            "mixed_minus_fp64_in_se": [
+                (res[i].estimates[0].value - f64[i].estimates[0].value) / f64[i].estimates[0].standard_error
+                for i in range(2)],
+            "se_ratio_mixed_over_fp64": [res[i].estimates[0].standard_error / f64[i].estimates[0].standard_error
+                                         for i in range(2)],
+        }
+    if not args.no_cpu:
+        cb = cpu_baseline(reqs, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["speedup_vs_cpu_all_cores"] = value / cb["value"]
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,434 @@
+// C ABI of the engine (include/pe.h): plan lifecycle, launches, block-partial
+// merging and result assembly.
+//
+// Result assembly mirrors execute_pseudo / execute_rqmc and
+// estimate_from_statistics (crates/pricing/src/engine/risk/valuation.rs:126-437,
+// crates/pricing/src/engine/risk/report.rs:454-473): pseudo-MC statistics over
+// sampling units, RQMC statistics as a two-pass over per-scramble means.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "pe.h"
+#include "pe_device.hpp"
+#include "pe_plan.hpp"
+
+struct pe_plan {
+    pe::HostPlan host;
+};
+
+namespace pe {
+namespace {
+
+constexpr double kNormal95 = 1.959963984540054;  // crates/pricing/src/engine/plan/simulation.rs:24
+
+thread_local std::string g_error;
+thread_local double g_kernel_ms = 0.0;
+thread_local uint32_t g_kernel_launches = 0;
+
+std::mutex g_init_mutex;
+bool g_initialised = false;
+hipStream_t g_stream = nullptr;
+int g_device = 0;
+
+pe_status fail(pe_status s, const std::string& what) {
+    g_error = what;
+    return s;
+}
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw PlanError(PE_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+struct Stat {  // Neumaier sum + centred moments (reduction.rs:3-161)
+    double sum = 0.0, corr = 0.0, mean = 0.0, m2 = 0.0;
+    uint64_t n = 0;
+
+    void add_sum(double v) {
+        const double t = sum + v;
+        if (std::fabs(sum) >= std::fabs(v)) corr += (sum - t) + v; else corr += (v - t) + sum;
+        sum = t;
+    }
+    void merge(const BlockStat& b) {
+        if (b.n == 0) return;
+        add_sum(b.sum);
+        add_sum(b.corr);
+        if (n == 0) {
+            mean = b.mean; m2 = b.m2; n = b.n;
+            return;
+        }
+        const uint64_t nn = n + b.n;
+        const double delta = b.mean - mean;
+        const double lw = static_cast<double>(n), rw = static_cast<double>(b.n), tw = static_cast<double>(nn);
+        mean = mean + delta * (rw / tw);
+        m2 = (m2 + b.m2) + delta * delta * (lw * rw / tw);
+        n = nn;
+    }
+    double total() const { return sum + corr; }
+    static Stat two_pass(const std::vector<double>& v) {
+        Stat s;
+        for (double x : v) s.add_sum(x);
+        s.n = v.size();
+        if (v.empty()) return s;
+        s.mean = s.total() / static_cast<double>(v.size());
+        Stat q;
+        for (double x : v) {
+            const double d = x - s.mean;
+            q.add_sum(d * d);
+        }
+        s.m2 = q.total();
+        return s;
+    }
+};
+
+pe_estimate estimate(const Stat& s, uint64_t units) {
+    const double inv = 1.0 / static_cast<double>(units);
+    const double value = s.total() * inv;
+    const double var = s.n > 1 ? s.m2 / static_cast<double>(s.n - 1) : 0.0;
+    const double se = std::sqrt(var * inv);
+    return pe_estimate{value, se, value - kNormal95 * se, value + kNormal95 * se, units};
+}
+
+void ensure_init() {
+    std::lock_guard<std::mutex> lock(g_init_mutex);
+    if (g_initialised) return;
+    int count = 0;
+    check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+    if (count <= 0) throw PlanError(PE_ERR_DEVICE, "no HIP device");
+    check(hipSetDevice(g_device), "hipSetDevice");
+    check(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking), "hipStreamCreate");
+    g_initialised = true;
+}
+
+// Converts block partials to per-component statistics and fills the result.
+void assemble(const HostPlan& p, const std::vector<BlockStat>& partials, uint32_t blocks_x, uint32_t scrambles,
+              uint32_t comps, uint32_t comp_offset, uint32_t result_comps, pe_result* out) {
+    const pe_engine& e = p.req.engine;
+    const bool rqmc = e.kind == PE_ENGINE_RQMC;
+    const uint64_t independent = rqmc ? e.scrambles : e.units;
+    std::vector<Stat> stats(PE_COMPONENTS);
+    if (!rqmc) {
+        for (uint32_t c = 0; c < result_comps; ++c)
+            for (uint32_t b = 0; b < blocks_x; ++b) stats[c].merge(partials[static_cast<size_t>(b) * comps + comp_offset + c]);
+    } else {
+        for (uint32_t c = 0; c < result_comps; ++c) {
+            std::vector<double> means;
+            for (uint32_t s = 0; s < scrambles; ++s) {
+                Stat within;
+                for (uint32_t b = 0; b < blocks_x; ++b)
+                    within.merge(partials[(static_cast<size_t>(s) * blocks_x + b) * comps + comp_offset + c]);
+                means.push_back(within.total() / static_cast<double>(e.units));
+            }
+            stats[c] = Stat::two_pass(means);
+        }
+    }
+    for (int c = 0; c < PE_COMPONENTS; ++c) {
+        out->estimates[c] = estimate(stats[c], independent);
+        out->component_sum[c] = stats[c].total();
+        out->component_mean[c] = stats[c].mean;
+        out->component_m2[c] = stats[c].m2;
+    }
+    double sv = 0.0;
+    if (p.total_variance != 0.0) {
+        if (stats[0].n < 2) throw PlanError(PE_ERR_INSUFFICIENT_UNITS, "insufficient sampling units");
+        sv = stats[0].m2 / static_cast<double>(stats[0].n - 1);
+    }
+    out->sampling_variance = sv;
+    out->estimator_variance = sv / static_cast<double>(independent);
+    out->independent_units = independent;
+    const uint64_t mult = e.antithetic ? 2 : 1;
+    out->evaluated_paths = e.units * (rqmc ? e.scrambles : 1) * mult;
+    out->time_steps = p.lv ? static_cast<uint32_t>(p.nodes.size() - 1) : static_cast<uint32_t>(p.obs_times.size());
+    out->path_steps = out->evaluated_paths * out->time_steps;
+    out->sobol_dimension = rqmc ? p.sobol_dims : 0;
+}
+
+dim3 grid_for(const pe_engine& e) {
+    const uint64_t blocks = (e.units + kBlockThreads - 1) / kBlockThreads;
+    const uint32_t bx = static_cast<uint32_t>(std::min<uint64_t>(blocks, 1u << 16));
+    return dim3(bx, e.kind == PE_ENGINE_RQMC ? e.scrambles : 1u, 1u);
+}
+
+void ensure_sobol(HostPlan& p) {
+    const pe_engine& e = p.req.engine;
+    if (e.kind != PE_ENGINE_RQMC || p.d_sobol.ptr) return;
+    p.d_sobol.upload(build_sobol_table(e.seed, e.scrambles, p.sobol_dims));
+}
+
+template <typename F>
+float timed(F&& launch) {
+    hipEvent_t a, b;
+    check(hipEventCreate(&a), "hipEventCreate");
+    check(hipEventCreate(&b), "hipEventCreate");
+    check(hipEventRecord(a, g_stream), "hipEventRecord");
+    launch();
+    check(hipEventRecord(b, g_stream), "hipEventRecord");
+    check(hipEventSynchronize(b), "hipEventSynchronize");
+    float ms = 0.0f;
+    check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ms;
+}
+
+void evaluate_bs(HostPlan& p, pe_backend backend, pe_result* out) {
+    const pe_engine& e = p.req.engine;
+    ensure_sobol(p);
+    if (!p.d_obs.ptr) {
+        std::vector<ObsDesc> obs(p.obs_times.size());
+        double prev = 0.0;
+        for (size_t i = 0; i < obs.size(); ++i) {
+            ObsDesc& o = obs[i];
+            std::memset(&o, 0, sizeof(o));
+            o.time = p.obs_times[i];
+            o.sqrt_step = std::sqrt(std::max(o.time - prev, 0.0));
+            prev = o.time;
+            o.forward = p.obs_forwards[i];
+            o.weight[0] = p.obs_weight[i];
+            o.mask = p.obs_weight[i] != 0.0 ? 1u : 0u;
+        }
+        p.d_obs.upload(obs);
+    }
+    BsArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.obs = p.d_obs.as<ObsDesc>();
+    a.sobol = p.d_sobol.as<uint32_t>();
+    a.product = p.product;
+    a.seed = e.seed;
+    a.units = e.units;
+    a.s0 = p.spot;
+    a.expiry = p.expiry;
+    a.gamma_bump = p.gamma_bump;
+    a.spot_bump = p.spot_bump;
+    a.vol_bump = p.vol_bump;
+    a.n_obs = static_cast<uint32_t>(p.obs_times.size());
+    a.rqmc = e.kind == PE_ENGINE_RQMC;
+    a.antithetic = e.antithetic ? 1u : 0u;
+    a.risk = p.risk ? 1u : 0u;
+    a.single_terminal = a.n_obs == 1 ? 1u : 0u;
+    a.request_gamma = p.request_gamma ? 1u : 0u;
+    a.zero_variance = p.total_variance == 0.0 ? 1u : 0u;
+    a.sobol_dims = p.sobol_dims;
+    auto variant = [&](int k, double spot, double vol) {
+        a.variants[k] = Variant{spot, vol, spot / p.spot};
+    };
+    variant(0, p.spot, p.vol);
+    uint32_t nv = 1;
+    if (p.risk) {
+        variant(1, p.spot - p.spot_bump, p.vol);
+        variant(2, p.spot + p.spot_bump, p.vol);
+        nv = 3;
+        if (p.vol_bump != 0.0) {
+            variant(3, p.spot, p.vol - p.vol_bump);
+            variant(4, p.spot, p.vol + p.vol_bump);
+            nv = 5;
+        }
+    }
+    a.n_variants = nv;
+    const uint32_t comps = p.risk ? 10u : 1u;
+    a.components = comps;
+    const dim3 grid = grid_for(e);
+    DeviceBuffer partials;
+    partials.reserve(static_cast<size_t>(grid.x) * grid.y * comps * sizeof(BlockStat));
+    a.partials = partials.ptr;
+    const float ms = timed([&] {
+        check(backend == PE_BACKEND_GPU_FP64 ? launch_bs_f64(a, p.risk, grid, g_stream)
+                                             : launch_bs_f32(a, p.risk, grid, g_stream),
+              "bs kernel launch");
+    });
+    std::vector<BlockStat> host(static_cast<size_t>(grid.x) * grid.y * comps);
+    check(hipMemcpy(host.data(), partials.ptr, host.size() * sizeof(BlockStat), hipMemcpyDeviceToHost), "partials copy");
+    std::memset(out, 0, sizeof(*out));
+    assemble(p, host, grid.x, grid.y, comps, 0, comps, out);
+    out->device_ms = ms;
+    g_kernel_ms = ms;
+    g_kernel_launches = 1;
+}
+
+void upload_lv_tables(HostPlan& p) {
+    if (p.d_steps.ptr) return;
+    p.d_steps.upload(p.steps);
+    p.d_program.upload(p.program);
+    p.d_grid.upload(p.lv_v);
+    p.d_x.upload(p.lv_x);
+    std::vector<float> g32(p.lv_v.begin(), p.lv_v.end()), x32(p.lv_x.begin(), p.lv_x.end());
+    p.d_grid32.upload(g32);
+    p.d_x32.upload(x32);
+}
+
+// Evaluates local-vol plans that share one simulation grid in one launch.
+void evaluate_lv(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* outs) {
+    HostPlan& p0 = *plans[0];
+    const pe_engine& e = p0.req.engine;
+    if (plans.size() > static_cast<size_t>(kMaxProducts)) throw PlanError(PE_ERR_SIZE_LIMIT, "too many shared plans");
+    for (HostPlan* q : plans) {
+        if (!q->lv) throw PlanError(PE_ERR_INVALID_ARGUMENT, "shared evaluation needs local-vol plans");
+        if (q->nodes != p0.nodes || q->spot != p0.spot || q->node_forwards != p0.node_forwards ||
+            q->lv_v != p0.lv_v || q->lv_x != p0.lv_x || q->lv_t != p0.lv_t ||
+            std::memcmp(&q->req.engine, &e, sizeof(e)) != 0)
+            throw PlanError(PE_ERR_INVALID_ARGUMENT, "plans do not share one simulated path set");
+    }
+    ensure_sobol(p0);
+    upload_lv_tables(p0);
+    const size_t nodes = p0.nodes.size();
+    // Per-node fixing weights and the steps' node masks over all products.
+    std::vector<double> node_weight(nodes * kMaxProducts, 0.0);
+    std::vector<LvStep> steps = p0.steps;
+    LvArgs a;
+    std::memset(&a, 0, sizeof(a));
+    for (size_t k = 0; k < plans.size(); ++k) {
+        const HostPlan& q = *plans[k];
+        a.products[k] = q.product;
+        for (size_t i = 0; i < q.obs_nodes.size(); ++i) {
+            const uint32_t node = q.obs_nodes[i];
+            node_weight[node * kMaxProducts + k] = q.obs_weight[i];
+            const bool needed = q.obs_weight[i] != 0.0 || static_cast<int32_t>(node) == q.product.terminal_obs;
+            if (!needed) continue;
+            if (node > 0) steps[node - 1].node_mask |= 1 << k;
+            else a.node0_mask |= 1u << k;
+        }
+    }
+    DeviceBuffer d_steps, d_node_weight;
+    d_steps.upload(steps);
+    d_node_weight.upload(node_weight);
+    a.steps = d_steps.as<LvStep>();
+    a.program = p0.d_program.as<BridgeOp>();
+    a.grid = p0.d_grid.as<double>();
+    a.grid32 = p0.d_grid32.as<float>();
+    a.x_nodes = p0.d_x.as<double>();
+    a.x_nodes32 = p0.d_x32.as<float>();
+    a.sobol = p0.d_sobol.as<uint32_t>();
+    a.node_weight = d_node_weight.as<double>();
+    a.seed = e.seed;
+    a.units = e.units;
+    a.spot = p0.spot;
+    a.n_steps = static_cast<uint32_t>(p0.steps.size());
+    a.n_program = static_cast<uint32_t>(p0.program.size());
+    a.nt = static_cast<uint32_t>(p0.lv_t.size());
+    a.nx = static_cast<uint32_t>(p0.lv_x.size());
+    a.n_products = static_cast<uint32_t>(plans.size());
+    a.rqmc = e.kind == PE_ENGINE_RQMC;
+    a.antithetic = e.antithetic ? 1u : 0u;
+    a.bridge = e.brownian_bridge ? 1u : 0u;
+    a.depth = p0.bridge_depth;
+    a.sobol_dims = p0.sobol_dims;
+    const dim3 grid = grid_for(e);
+    const uint32_t comps = kMaxProducts;
+    DeviceBuffer partials, err;
+    partials.reserve(static_cast<size_t>(grid.x) * grid.y * comps * sizeof(BlockStat));
+    err.reserve(sizeof(int));
+    const int no_error = 0x7fffffff;
+    check(hipMemcpy(err.ptr, &no_error, sizeof(int), hipMemcpyHostToDevice), "error word init");
+    a.partials = partials.ptr;
+    const size_t real = backend == PE_BACKEND_GPU_FP64 ? sizeof(double) : sizeof(float);
+    const size_t grid_bytes = ((real * (a.nt * a.nx + a.nx)) + 15u) & ~static_cast<size_t>(15u);
+    const size_t lds = grid_bytes + static_cast<size_t>(std::max<uint32_t>(a.depth, 1u)) * kBlockThreads * sizeof(double);
+    if (lds > 160u * 1024u) throw PlanError(PE_ERR_SIZE_LIMIT, "local-vol grid and bridge stack exceed LDS");
+    const float ms = timed([&] {
+        check(backend == PE_BACKEND_GPU_FP64 ? launch_lv_f64(a, grid, lds, err.as<int>(), g_stream)
+                                             : launch_lv_f32(a, grid, lds, err.as<int>(), g_stream),
+              "lv kernel launch");
+    });
+    int failing = 0;
+    check(hipMemcpy(&failing, err.ptr, sizeof(int), hipMemcpyDeviceToHost), "error word copy");
+    if (failing != no_error) {
+        for (size_t k = 0; k < plans.size(); ++k) outs[k].failing_unit = static_cast<uint64_t>(failing);
+        throw PlanError(PE_ERR_NONFINITE_STATE,
+                        "local-vol Log-Euler produced a non-finite state at sampling unit " + std::to_string(failing));
+    }
+    std::vector<BlockStat> host(static_cast<size_t>(grid.x) * grid.y * comps);
+    check(hipMemcpy(host.data(), partials.ptr, host.size() * sizeof(BlockStat), hipMemcpyDeviceToHost), "partials copy");
+    for (size_t k = 0; k < plans.size(); ++k) {
+        std::memset(&outs[k], 0, sizeof(pe_result));
+        assemble(*plans[k], host, grid.x, grid.y, comps, static_cast<uint32_t>(k), 1, &outs[k]);
+        outs[k].device_ms = ms;
+    }
+    g_kernel_ms = ms;
+    g_kernel_launches = 1;
+}
+
+template <typename F>
+pe_status guarded(F&& f) {
+    try {
+        f();
+        return PE_OK;
+    } catch (const PlanError& e) {
+        return fail(e.status, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(PE_ERR_ALLOCATION, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(PE_ERR_INVALID_ARGUMENT, e.what());
+    }
+}
+
+}  // namespace
+}  // namespace pe
+
+using namespace pe;
+
+extern "C" {
+
+int32_t pe_abi_version(void) { return PE_ABI_VERSION; }
+
+const char* pe_last_error(void) { return g_error.c_str(); }
+
+pe_status pe_init(int32_t device) {
+    return guarded([&] {
+        g_device = device;
+        ensure_init();
+    });
+}
+
+pe_status pe_plan_compile(const pe_request* request, const pe_exec_policy* policy, pe_plan** out) {
+    if (!request || !out) return fail(PE_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    return guarded([&] {
+        std::unique_ptr<pe_plan> plan(new pe_plan());
+        compile_host_plan(plan->host, *request, policy);
+        *out = plan.release();
+    });
+}
+
+void pe_plan_free(pe_plan* plan) { delete plan; }
+
+pe_status pe_plan_evaluate(const pe_plan* plan, pe_backend backend, pe_result* out) {
+    if (!plan || !out) return fail(PE_ERR_INVALID_ARGUMENT, "null argument");
+    if (backend != PE_BACKEND_GPU_FP64 && backend != PE_BACKEND_GPU_MIXED) return fail(PE_ERR_INVALID_ARGUMENT, "backend");
+    return guarded([&] {
+        ensure_init();
+        HostPlan& p = const_cast<pe_plan*>(plan)->host;
+        if (p.lv) {
+            std::vector<HostPlan*> plans{&p};
+            evaluate_lv(plans, backend, out);
+        } else {
+            evaluate_bs(p, backend, out);
+        }
+    });
+}
+
+pe_status pe_plans_evaluate_shared(const pe_plan* const* plans, uint32_t count, pe_backend backend, pe_result* out) {
+    if (!plans || !out || count == 0) return fail(PE_ERR_INVALID_ARGUMENT, "null argument");
+    if (backend != PE_BACKEND_GPU_FP64 && backend != PE_BACKEND_GPU_MIXED) return fail(PE_ERR_INVALID_ARGUMENT, "backend");
+    return guarded([&] {
+        ensure_init();
+        std::vector<HostPlan*> hp;
+        for (uint32_t i = 0; i < count; ++i) {
+            if (!plans[i]) throw PlanError(PE_ERR_INVALID_ARGUMENT, "null plan");
+            hp.push_back(&const_cast<pe_plan*>(plans[i])->host);
+        }
+        evaluate_lv(hp, backend, out);
+    });
+}
+
+pe_status pe_last_kernel_stats(double* avg_ms, uint32_t* launches) {
+    if (avg_ms) *avg_ms = g_kernel_ms;
+    if (launches) *launches = g_kernel_launches;
+    return PE_OK;
+}
+
+}  // extern "C"

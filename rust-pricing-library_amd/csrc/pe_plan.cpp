@@ -1,0 +1,435 @@
+// Host FP64 plan compilation for the engine.
+//
+// Mirrors SimulationPlan::compile (crates/pricing/src/engine/compile/simulation.rs:78-640)
+// for the single-asset products on the hot path, the local-vol runtime
+// (crates/pricing/src/engine/compile/local_vol.rs:14-103, processes/local_vol.rs:166-237),
+// the RQMC scramble table (crates/pricing/src/engine/sampling/qmc.rs:190-369) and the
+// Brownian-bridge instruction order (crates/pricing/src/engine/sampling/bridge.rs:164-467).
+// All arithmetic is FP64 with the reference's expressions; the device tables
+// carry the results.
+#include "pe_plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <queue>
+
+#include "pe_device.hpp"
+
+namespace pe {
+
+namespace {
+
+constexpr double kRelativeSpotBump = 1.0e-4;  // crates/pricing/src/api/mc_result.rs:288
+constexpr double kVolatilityBump = 1.0e-4;    // crates/pricing/src/api/mc_result.rs:289
+
+[[noreturn]] void invalid(const std::string& what) { throw PlanError(PE_ERR_INVALID_ARGUMENT, what); }
+[[noreturn]] void unsupported(const std::string& what) { throw PlanError(PE_ERR_UNSUPPORTED, what); }
+
+// Index of the last node ordered at or before `value`, clamped to the last cell.
+size_t lower_cell(const std::vector<double>& nodes, double value) {
+    const size_t count = static_cast<size_t>(std::upper_bound(nodes.begin(), nodes.end(), value) - nodes.begin());
+    const size_t idx = count == 0 ? 0 : count - 1;
+    return std::min(idx, nodes.size() - 2);
+}
+
+void check_sorted(const std::vector<double>& v, bool reject_negative, const char* what) {
+    if (v.size() < 2) invalid(std::string(what) + ": needs at least two nodes");
+    for (size_t i = 0; i < v.size(); ++i) {
+        if (!std::isfinite(v[i]) || (reject_negative && v[i] < 0.0)) invalid(std::string(what) + ": invalid node");
+        if (i > 0 && !(v[i] > v[i - 1])) invalid(std::string(what) + ": nodes must increase");
+    }
+}
+
+// Sorted unique fixing days on or after valuation (CompiledPayoff::terminal_observations,
+// crates/pricing/src/engine/payoff/tape.rs:229-247).
+std::vector<int32_t> fixing_days(const pe_product& p) {
+    std::vector<int32_t> d;
+    auto add = [&](int32_t day) { if (day >= 0) d.push_back(day); };
+    switch (p.kind) {
+        case PE_PRODUCT_EUROPEAN:
+        case PE_PRODUCT_DIGITAL:
+            add(p.expiry_day);
+            break;
+        case PE_PRODUCT_ASIAN:
+            for (uint32_t i = 0; i < p.date_count; ++i)
+                if (!(p.fixings && !std::isnan(p.fixings[i]))) add(p.dates[i]);
+            break;
+        case PE_PRODUCT_BARRIER:
+            add(p.expiry_day);
+            if (!(p.continuous && p.historical_hit == 1))
+                for (uint32_t i = 0; i < p.date_count; ++i) add(p.dates[i]);
+            break;
+        case PE_PRODUCT_LOOKBACK:
+            for (uint32_t i = 0; i < p.date_count; ++i) add(p.dates[i]);
+            break;
+        default:
+            unsupported("product kind");
+    }
+    std::sort(d.begin(), d.end());
+    d.erase(std::unique(d.begin(), d.end()), d.end());
+    return d;
+}
+
+}  // namespace
+
+void HostCurve::init(const pe_curve& c) {
+    if (c.count < 2 || !c.times || !c.discount_factors) invalid("curve needs two pillars");
+    times.assign(c.times, c.times + c.count);
+    if (times[0] != 0.0) invalid("curve must start at t = 0");
+    if (c.discount_factors[0] != 1.0) invalid("curve must start at DF = 1");
+    log_dfs.resize(c.count);
+    for (uint32_t i = 0; i < c.count; ++i) {
+        if (!std::isfinite(times[i]) || times[i] < 0.0) invalid("curve pillar time");
+        if (!std::isfinite(c.discount_factors[i]) || c.discount_factors[i] <= 0.0) invalid("discount factor");
+        if (i > 0 && !(times[i] > times[i - 1])) invalid("curve pillars must increase");
+        log_dfs[i] = std::log(c.discount_factors[i]);
+    }
+    times[0] = 0.0;
+}
+
+double HostCurve::log_discount(double t) const {
+    if (!std::isfinite(t) || t < 0.0) invalid("curve query time");
+    if (t == 0.0) t = 0.0;
+    const auto it = std::lower_bound(times.begin(), times.end(), t);
+    const size_t right = static_cast<size_t>(it - times.begin());
+    auto segment = [&](size_t l, size_t r) {
+        const double w = (t - times[l]) / (times[r] - times[l]);
+        return log_dfs[l] + w * (log_dfs[r] - log_dfs[l]);
+    };
+    if (it != times.end() && *it == t) return log_dfs[right];
+    if (right < times.size()) return segment(right - 1, right);
+    return segment(times.size() - 2, times.size() - 1);
+}
+
+// EquityForward::evaluate (crates/pricing/src/market/forward.rs:129-161), no discrete dividends.
+double HostPlan::forward_at(double t) const {
+    const double f = spot * std::exp(dividend_curve.log_discount(t) - discount_curve.log_discount(t));
+    if (!std::isfinite(f) || f <= 0.0) invalid("non-finite forward");
+    return f;
+}
+
+// ----------------------------------------------------------- bridge order --
+void compile_bridge_program(const std::vector<double>& t, std::vector<BridgeOp>& program,
+                            std::vector<double>& inv_sqrt_dt, uint32_t& depth) {
+    if (t.size() < 2) invalid("Brownian bridge needs two nodes");
+    if (t[0] != 0.0 || std::signbit(t[0])) invalid("Brownian bridge must start at +0");
+    for (size_t i = 0; i + 1 < t.size(); ++i)
+        if (!(t[i + 1] > t[i]) || !std::isfinite(t[i + 1])) invalid("Brownian bridge times must increase");
+    inv_sqrt_dt.resize(t.size() - 1);
+    for (size_t i = 0; i + 1 < t.size(); ++i) inv_sqrt_dt[i] = 1.0 / std::sqrt(t[i + 1] - t[i]);
+
+    struct Cand {
+        double variance, time;
+        size_t node, left, right;
+    };
+    // Ord of IntervalCandidate (bridge.rs:137-146): variance, then earlier time, then lower node.
+    auto less = [](const Cand& a, const Cand& b) {
+        if (a.variance != b.variance) return a.variance < b.variance;
+        if (a.time != b.time) return a.time > b.time;
+        return a.node > b.node;
+    };
+    auto candidate = [&](size_t l, size_t r, Cand* out) {
+        if (r <= l + 1) return false;
+        const double den = t[r] - t[l];
+        bool have = false;
+        Cand best{};
+        for (size_t m = l + 1; m < r; ++m) {
+            const double var = ((t[m] - t[l]) * (t[r] - t[m])) / den;
+            if (!std::isfinite(var) || var <= 0.0) invalid("Brownian bridge conditional variance");
+            const Cand c{var, t[m], m, l, r};
+            if (!have || !less(c, best)) { best = c; have = true; }
+        }
+        *out = best;
+        return true;
+    };
+    struct Split {
+        size_t node;
+        uint32_t rank;
+        double lw, rw, sd;
+    };
+    std::map<std::pair<size_t, size_t>, Split> splits;
+    const size_t n = t.size() - 1;
+    std::priority_queue<Cand, std::vector<Cand>, decltype(less)> heap(less);
+    Cand c;
+    if (candidate(0, n, &c)) heap.push(c);
+    uint32_t rank = 1;
+    while (!heap.empty()) {
+        c = heap.top();
+        heap.pop();
+        const double den = t[c.right] - t[c.left];
+        splits[{c.left, c.right}] = Split{c.node, rank++, (t[c.right] - t[c.node]) / den,
+                                          (t[c.node] - t[c.left]) / den, std::sqrt(c.variance)};
+        Cand a, b;
+        if (candidate(c.left, c.node, &a)) heap.push(a);
+        if (candidate(c.node, c.right, &b)) heap.push(b);
+    }
+    // In-order traversal of the interval tree: values come out in time order.
+    program.clear();
+    program.push_back(BridgeOp{1, 0, 1, 0, 0.0, 0.0, std::sqrt(t[n])});
+    std::vector<size_t> stack{n};
+    depth = 1;
+    size_t left = 0;
+    while (!stack.empty()) {
+        const size_t right = stack.back();
+        if (right == left + 1) {
+            program.push_back(BridgeOp{0, 0, 0, static_cast<int32_t>(left), 0.0, 0.0, 0.0});
+            stack.pop_back();
+            left = right;
+        } else {
+            const auto it = splits.find({left, right});
+            if (it == splits.end()) invalid("Brownian bridge interval without a split");
+            const Split& s = it->second;
+            program.push_back(BridgeOp{1, static_cast<int32_t>(s.rank), 0, 0, s.lw, s.rw, s.sd});
+            stack.push_back(s.node);
+            depth = std::max<uint32_t>(depth, static_cast<uint32_t>(stack.size()));
+        }
+    }
+}
+
+// ------------------------------------------------------------ Sobol table --
+// [scramble][dim][33]: LMS-scrambled direction words (MSB-first unit lower
+// triangular rows from Philox domain RqmcScramble) and the digital shift.
+std::vector<uint32_t> build_sobol_table(uint64_t seed, uint32_t scrambles, uint32_t dims) {
+    uint32_t available = 0;
+    const uint32_t* words = direction_words(&available);
+    if (dims == 0) invalid("zero Sobol dimension");
+    if (dims > available) throw PlanError(PE_ERR_SIZE_LIMIT, "Sobol dimension exceeds the Joe--Kuo limit");
+    std::vector<uint32_t> table(static_cast<size_t>(scrambles) * dims * 33u);
+    auto word = [seed](uint64_t scramble, uint32_t dim) {
+        const U4 b = philox_block(seed, scramble, dim >> 2, 2u);
+        return lane_of(b, dim & 3u);
+    };
+    for (uint32_t s = 0; s < scrambles; ++s) {
+        for (uint32_t d = 0; d < dims; ++d) {
+            uint32_t rows[32];
+            const uint32_t block = d * 32u;
+            const uint32_t shift = word(s, block);
+            rows[0] = 0x80000000u;
+            for (uint32_t r = 1; r < 32; ++r) {
+                const uint32_t raw = word(s, block + r);
+                rows[r] = (raw & (0xffffffffu << (32 - r))) | (1u << (31 - r));
+            }
+            uint32_t* out = &table[(static_cast<size_t>(s) * dims + d) * 33u];
+            const uint32_t* dir = words + static_cast<size_t>(d) * 32u;
+            for (int k = 0; k < 32; ++k) {
+                uint32_t v = 0;
+                for (int r = 0; r < 32; ++r) v |= (static_cast<uint32_t>(__builtin_popcount(dir[k] & rows[r])) & 1u) << (31 - r);
+                out[k] = v;
+            }
+            out[32] = shift;
+        }
+    }
+    return table;
+}
+
+// ------------------------------------------------------------------ plan --
+void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* policy) {
+    p.req = r;
+    const pe_product& src = r.product;
+    if (src.date_count) {
+        if (!src.dates) invalid("product dates missing");
+        p.dates.assign(src.dates, src.dates + src.date_count);
+        if (src.weights) p.weights.assign(src.weights, src.weights + src.date_count);
+        if (src.fixings) p.fixings.assign(src.fixings, src.fixings + src.date_count);
+    }
+    p.req.product.dates = p.dates.empty() ? nullptr : p.dates.data();
+    p.req.product.weights = p.weights.empty() ? nullptr : p.weights.data();
+    p.req.product.fixings = p.fixings.empty() ? nullptr : p.fixings.data();
+    const pe_product& pr = p.req.product;
+    if (policy && policy->reduction_block_size) p.block_size = policy->reduction_block_size;
+
+    const pe_market& m = r.market;
+    if (!std::isfinite(m.spot) || m.spot <= 0.0) invalid("spot must be positive");
+    if (m.dividend_count) unsupported("discrete dividends");
+    p.spot = m.spot;
+    p.discount_curve.init(m.discount_curve);
+    p.dividend_curve.init(m.dividend_curve);
+
+    if (pr.kind == PE_PRODUCT_BARRIER && pr.continuous) unsupported("continuous barrier monitoring");
+    if (!(pr.strike > 0.0) || !(pr.notional > 0.0)) invalid("strike and notional must be positive");
+    for (size_t i = 1; i < p.dates.size(); ++i)
+        if (!(p.dates[i] > p.dates[i - 1])) invalid("product dates must increase strictly");
+    p.obs_days = fixing_days(pr);
+    if (p.obs_days.empty()) unsupported("fully fixed payoffs");
+    p.obs_times.clear();
+    for (int32_t d : p.obs_days) p.obs_times.push_back(static_cast<double>(d) / 365.0);
+    p.expiry = static_cast<double>(pr.expiry_day) / 365.0;
+    p.forward = p.forward_at(p.expiry);
+    p.discount = std::exp(p.discount_curve.log_discount(static_cast<double>(pr.payment_day) / 365.0));
+
+    auto obs_index = [&](int32_t day) -> int32_t {
+        const auto it = std::lower_bound(p.obs_days.begin(), p.obs_days.end(), day);
+        if (it == p.obs_days.end() || *it != day) invalid("missing observation");
+        return static_cast<int32_t>(it - p.obs_days.begin());
+    };
+
+    // Product descriptor and per-observation fixing weights.
+    ProductDesc& d = p.product;
+    std::memset(&d, 0, sizeof(d));
+    d.call = pr.side == PE_CALL ? 1 : 0;
+    d.strike = pr.strike;
+    d.notional = pr.notional;
+    d.discount = p.discount;
+    d.smoothing = r.risk.smoothing ? 1 : 0;
+    d.half_width = r.risk.smoothing_half_width;
+    if (d.smoothing && !(d.half_width > 0.0)) invalid("smoothing half width must be positive");
+    d.terminal_obs = -1;
+    p.obs_weight.assign(p.obs_days.size(), 0.0);
+    switch (pr.kind) {
+        case PE_PRODUCT_EUROPEAN:
+            d.kind = kEuropean;
+            d.terminal_obs = obs_index(pr.expiry_day);
+            break;
+        case PE_PRODUCT_DIGITAL:
+            d.kind = kDigital;
+            d.payout = pr.payout;
+            d.payout_asset = pr.payout_is_asset ? 1 : 0;
+            d.terminal_obs = obs_index(pr.expiry_day);
+            break;
+        case PE_PRODUCT_ASIAN: {
+            d.kind = kAsian;
+            if (!pr.weights) invalid("Asian weights missing");
+            // The tape adds fixings in date order from 0.0; known fixings must
+            // precede future ones so their sum is a prefix.
+            double prefix = 0.0;
+            bool seen_unknown = false;
+            for (uint32_t i = 0; i < pr.date_count; ++i) {
+                const bool known = pr.fixings && !std::isnan(pr.fixings[i]);
+                if (known) {
+                    if (seen_unknown) unsupported("known Asian fixing after a future fixing");
+                    prefix = prefix + pr.fixings[i] * pr.weights[i];
+                } else {
+                    if (pr.dates[i] < 0) invalid("past Asian observation without a fixing");
+                    seen_unknown = true;
+                    p.obs_weight[obs_index(pr.dates[i])] = pr.weights[i];
+                }
+            }
+            d.asian_prefix = prefix;
+            break;
+        }
+        case PE_PRODUCT_BARRIER:
+            d.kind = kBarrier;
+            d.barrier = pr.barrier;
+            d.down = pr.direction_down ? 1 : 0;
+            d.knock_in = pr.knock_in ? 1 : 0;
+            d.has_rebate = pr.has_rebate ? 1 : 0;
+            d.rebate = pr.rebate;
+            d.initial_hit = pr.historical_hit == 1 ? 1 : 0;
+            d.terminal_obs = obs_index(pr.expiry_day);
+            for (uint32_t i = 0; i < pr.date_count; ++i)
+                if (pr.dates[i] >= 0) p.obs_weight[obs_index(pr.dates[i])] = 1.0;
+            break;
+        case PE_PRODUCT_LOOKBACK:
+            d.kind = kLookback;
+            d.has_hist_ext = pr.has_historical_extremum ? 1 : 0;
+            d.hist_ext = pr.historical_extremum;
+            for (uint32_t i = 0; i < pr.date_count; ++i)
+                if (pr.dates[i] >= 0) p.obs_weight[obs_index(pr.dates[i])] = 1.0;
+            break;
+        default:
+            unsupported("product kind");
+    }
+
+    const pe_model& model = r.model;
+    if (model.kind == PE_MODEL_BLACK_SCHOLES || model.kind == PE_MODEL_BLACK76) {
+        p.vol = model.volatility;
+        if (!std::isfinite(p.vol) || p.vol < 0.0) invalid("volatility must be finite and non-negative");
+        p.total_variance = p.vol * p.vol * p.expiry;
+        for (double t : p.obs_times) p.obs_forwards.push_back(p.forward_at(t));
+    } else if (model.kind == PE_MODEL_LOCAL_VOL) {
+        p.lv = true;
+        const pe_local_variance_grid& g = model.lv;
+        if (!g.time_nodes || !g.x_nodes || !g.values) invalid("local-variance grid arrays missing");
+        p.lv_t.assign(g.time_nodes, g.time_nodes + g.time_count);
+        p.lv_x.assign(g.x_nodes, g.x_nodes + g.x_count);
+        p.lv_v.assign(g.values, g.values + static_cast<size_t>(g.time_count) * g.x_count);
+        check_sorted(p.lv_t, true, "local-variance time nodes");
+        check_sorted(p.lv_x, false, "local-variance log-moneyness nodes");
+        if (!std::isfinite(g.floor) || g.floor <= 0.0 || !std::isfinite(g.cap) || g.cap < g.floor)
+            invalid("local-variance floor/cap");
+        for (double v : p.lv_v)
+            if (!std::isfinite(v) || v < g.floor || v > g.cap) invalid("local-variance value outside [floor, cap]");
+        if (p.lv_t.front() != 0.0 || p.lv_t.back() != p.expiry)
+            invalid("local-variance grid must span [0, expiry]");
+        p.lv_cap = g.cap;
+        double max_step = 0.0;
+        for (size_t i = 0; i + 1 < p.lv_t.size(); ++i) max_step = std::max(max_step, p.lv_t[i + 1] - p.lv_t[i]);
+        // LocalVolTimeGrid::compile: union of grid nodes and events, ceil(dt / max) substeps.
+        std::vector<double> events = p.lv_t;
+        events.insert(events.end(), p.obs_times.begin(), p.obs_times.end());
+        std::sort(events.begin(), events.end());
+        events.erase(std::unique(events.begin(), events.end()), events.end());
+        if (events[0] != 0.0) events.insert(events.begin(), 0.0);
+        p.nodes.assign(1, 0.0);
+        for (size_t i = 0; i + 1 < events.size(); ++i) {
+            const double l = events[i], rr = events[i + 1];
+            const double interval = rr - l;
+            const double raw = std::ceil(interval / max_step);
+            if (!std::isfinite(raw) || raw > 1e9) throw PlanError(PE_ERR_SIZE_LIMIT, "local-vol step count");
+            const size_t steps = std::max<size_t>(1, static_cast<size_t>(raw));
+            const double sub = interval / static_cast<double>(steps);
+            for (size_t k = 1; k < steps; ++k) p.nodes.push_back(l + sub * static_cast<double>(k));
+            p.nodes.push_back(rr);
+        }
+        for (double t : p.nodes) p.node_forwards.push_back(p.forward_at(t));
+        p.obs_nodes.clear();
+        for (double t : p.obs_times) {
+            const auto it = std::find(p.nodes.begin(), p.nodes.end(), t);
+            if (it == p.nodes.end()) invalid("observation missing from the local-vol grid");
+            p.obs_nodes.push_back(static_cast<uint32_t>(it - p.nodes.begin()));
+        }
+        p.total_variance = p.nodes.back() * p.lv_cap;
+        const size_t n = p.nodes.size() - 1;
+        std::vector<double> inv_sqrt_dt;
+        if (r.engine.brownian_bridge) {
+            compile_bridge_program(p.nodes, p.program, inv_sqrt_dt, p.bridge_depth);
+            if (p.bridge_depth > static_cast<uint32_t>(kMaxBridgeDepth))
+                throw PlanError(PE_ERR_SIZE_LIMIT, "Brownian-bridge tree deeper than the device stack");
+        } else {
+            inv_sqrt_dt.assign(n, 0.0);
+        }
+        p.steps.resize(n);
+        for (size_t s = 0; s < n; ++s) {
+            LvStep& st = p.steps[s];
+            st.t = p.nodes[s];
+            st.dt = p.nodes[s + 1] - st.t;
+            st.sqrt_dt = std::sqrt(st.dt);
+            st.ratio = p.node_forwards[s + 1] / p.node_forwards[s];
+            st.inv_sqrt_dt = inv_sqrt_dt[s];
+            st.forward = p.node_forwards[s];
+            st.next_forward = p.node_forwards[s + 1];
+            const size_t tc = lower_cell(p.lv_t, st.t);
+            st.t_cell = static_cast<int32_t>(tc);
+            st.t_weight = (st.t - p.lv_t[tc]) / (p.lv_t[tc + 1] - p.lv_t[tc]);
+            st.node_mask = 0;
+        }
+        // node-indexed terminal for the LV kernel
+        if (d.terminal_obs >= 0) d.terminal_obs = static_cast<int32_t>(p.obs_nodes[d.terminal_obs]);
+    } else {
+        unsupported("model kind");
+    }
+
+    const uint64_t independent = r.engine.kind == PE_ENGINE_PSEUDO ? r.engine.units : r.engine.scrambles;
+    if (r.engine.kind != PE_ENGINE_PSEUDO && r.engine.kind != PE_ENGINE_RQMC) invalid("engine kind");
+    if (r.engine.units == 0) invalid("zero sampling units");
+    if (r.engine.kind == PE_ENGINE_RQMC && (r.engine.scrambles == 0 || r.engine.units > 0xffffffffull))
+        invalid("RQMC needs scrambles and at most 2^32 points");
+    if (p.total_variance > 0.0 && independent < 2)
+        throw PlanError(PE_ERR_INSUFFICIENT_UNITS, "at least two independent sampling units are required");
+
+    p.request_gamma = r.risk.gamma != 0;
+    p.risk = r.risk.delta || r.risk.vega || p.request_gamma;
+    if (p.lv && p.risk) unsupported("local-vol pathwise risk");
+    if (p.risk && !p.lv && (pr.kind == PE_PRODUCT_DIGITAL || pr.kind == PE_PRODUCT_BARRIER) && !d.smoothing)
+        invalid("pathwise risk of a discontinuous payoff needs compact-C2 smoothing");
+    if (p.request_gamma) p.gamma_bump = r.risk.gamma_relative ? p.spot * r.risk.gamma_bump : r.risk.gamma_bump;
+    if (p.request_gamma && (!std::isfinite(p.gamma_bump) || !(p.gamma_bump > 0.0) || p.gamma_bump >= p.spot))
+        invalid("Gamma bump must be positive and below the spot");
+    p.spot_bump = p.request_gamma ? p.gamma_bump : std::min(p.spot * kRelativeSpotBump, 0.5 * p.spot);
+    p.vol_bump = p.vol == 0.0 ? 0.0 : std::min(kVolatilityBump, p.vol * 0.5);
+    p.sobol_dims = p.lv ? static_cast<uint32_t>(p.nodes.size() - 1) : static_cast<uint32_t>(p.obs_times.size());
+}
+
+}  // namespace pe

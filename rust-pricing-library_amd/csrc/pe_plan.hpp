@@ -1,0 +1,117 @@
+// Host-side plan of the engine: FP64 plan compilation mirroring
+// SimulationPlan::compile (crates/pricing/src/engine/compile/simulation.rs:78-640,
+// crates/pricing/src/engine/compile/local_vol.rs:14-103), flattened into the
+// POD tables the kernels read.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pe.h"
+#include "pe_types.hpp"
+
+namespace pe {
+
+struct PlanError : std::runtime_error {
+    pe_status status;
+    PlanError(pe_status s, const std::string& what) : std::runtime_error(what), status(s) {}
+};
+
+// Device buffer owned by a plan.
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer() { release(); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    void reserve(size_t n) {
+        if (n <= bytes) return;
+        release();
+        if (hipMalloc(&ptr, n) != hipSuccess) {
+            ptr = nullptr;
+            throw PlanError(PE_ERR_ALLOCATION, "hipMalloc failed");
+        }
+        bytes = n;
+    }
+    template <typename T>
+    void upload(const std::vector<T>& v) {
+        reserve(v.size() * sizeof(T) + 16);
+        if (!v.empty() && hipMemcpy(ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+            throw PlanError(PE_ERR_DEVICE, "hipMemcpy to device failed");
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Log-linear discount curve (crates/pricing/src/market/curve.rs:39-212).
+struct HostCurve {
+    std::vector<double> times, log_dfs;
+    void init(const pe_curve& c);
+    double log_discount(double t) const;
+};
+
+struct HostPlan {
+    pe_request req{};
+    std::vector<int32_t> dates;
+    std::vector<double> weights, fixings;
+
+    HostCurve discount_curve, dividend_curve;
+    double spot = 0.0;
+    std::vector<int32_t> obs_days;
+    std::vector<double> obs_times, obs_forwards;
+    double expiry = 0.0, forward = 0.0, discount = 1.0;
+    double vol = 0.0, total_variance = 0.0;
+    double gamma_bump = 0.0, spot_bump = 0.0, vol_bump = 0.0;
+    bool risk = false, request_gamma = false;
+    uint64_t block_size = 4096;
+
+    // local vol
+    bool lv = false;
+    std::vector<double> lv_t, lv_x, lv_v;
+    double lv_cap = 0.0;
+    std::vector<double> nodes;          // simulation grid
+    std::vector<double> node_forwards;
+    std::vector<uint32_t> obs_nodes;    // node index per observation
+    std::vector<LvStep> steps;
+    std::vector<BridgeOp> program;
+    uint32_t bridge_depth = 0;
+
+    ProductDesc product{};
+    std::vector<double> obs_weight;     // per observation (Asian weight or 1 for fixings)
+
+    // RQMC
+    uint32_t sobol_dims = 0;
+
+    // device tables
+    DeviceBuffer d_obs, d_sobol, d_steps, d_program, d_grid, d_grid32, d_x, d_x32, d_node_weight;
+
+    double forward_at(double t) const;
+};
+
+void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* policy);
+std::vector<uint32_t> build_sobol_table(uint64_t seed, uint32_t scrambles, uint32_t dims);
+void compile_bridge_program(const std::vector<double>& times, std::vector<BridgeOp>& program,
+                            std::vector<double>& inv_sqrt_dt, uint32_t& depth);
+const uint32_t* direction_words(uint32_t* dimensions);  // embedded Joe--Kuo asset (host order)
+
+// Launchers (pe_kernels_f64.hip / pe_kernels_f32.hip).
+hipError_t launch_bs_f64(const BsArgs& a, bool risk, dim3 grid, hipStream_t s);
+hipError_t launch_bs_f32(const BsArgs& a, bool risk, dim3 grid, hipStream_t s);
+hipError_t launch_lv_f64(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s);
+hipError_t launch_lv_f32(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s);
+hipError_t launch_dupire(const double* theta_t, const double* theta_v, const double* theta_d, uint32_t nk,
+                         double slope, double rho, double eta, double gamma, const double* t, uint32_t nt,
+                         const double* x, uint32_t nx, double floor, double cap, double* out, uint32_t* repairs,
+                         int* status, hipStream_t s);
+
+}  // namespace pe

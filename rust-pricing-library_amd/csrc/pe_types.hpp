@@ -1,0 +1,129 @@
+// Plain-data descriptors shared by the host plan compiler and the kernels.
+// Everything here is POD so it can be passed by value as a kernel argument or
+// copied to HBM as a table.
+#pragma once
+
+#include <stdint.h>
+
+namespace pe {
+
+constexpr int kMaxProducts = 4;      // products sharing one simulated path set
+constexpr int kMaxVariants = 7;      // base, Gamma +/-, validation spot +/-, vol +/-
+constexpr int kMaxBridgeDepth = 64;  // streaming Brownian-bridge stack (LDS)
+constexpr int kBlockThreads = 256;
+
+enum ProductKind : int32_t { kEuropean = 0, kDigital = 1, kBarrier = 2, kAsian = 3, kLookback = 4 };
+
+// One product's payoff, specialised per family (the reference builds these as
+// SSA graphs, crates/pricing/src/engine/payoff/compile.rs:35-474).
+struct ProductDesc {
+    int32_t kind;
+    int32_t call;            // 1 call, 0 put
+    int32_t payout_asset;    // digital asset-or-nothing
+    int32_t down;            // barrier direction
+    int32_t knock_in;
+    int32_t has_rebate;
+    int32_t smoothing;       // compact-C2 smoothing of indicators
+    int32_t initial_hit;     // barrier: historical hit -> 1
+    int32_t has_hist_ext;    // lookback historical extremum
+    int32_t terminal_obs;    // observation index of the expiry fixing
+    int32_t first_obs;       // lookback: first future observation
+    int32_t pad;
+    double strike, notional, payout, barrier, rebate, half_width;
+    double discount;         // DF(payment)
+    double hist_ext;         // lookback historical extremum
+    double asian_prefix;     // Asian: known-fixing prefix sum
+};
+
+// One observation (BS kernel) or one grid node (LV kernel).  mask bit p: the
+// node is a fixing of product p; weight[p] is its Asian weight.
+struct ObsDesc {
+    double time;         // year fraction
+    double sqrt_step;    // sqrt(max(t_i - t_{i-1}, 0)) (BS path)
+    double forward;      // F(t_i)
+    double weight[kMaxProducts];
+    uint32_t mask;
+    uint32_t pad;
+};
+
+// A pathwise evaluation point (spot, vol) of the BS risk vector
+// (crates/pricing/src/engine/processes/black_scholes.rs:417-472).
+struct Variant {
+    double spot;
+    double vol;
+    double spot_scale;   // spot / S0
+};
+
+// Streaming bridge program: PUSH computes the value at a split node from the
+// current left value and the stack top; POP emits the next grid node.
+struct BridgeOp {
+    int32_t push;        // 1 push, 0 pop
+    int32_t rank;        // normal rank (Sobol / Philox dimension)
+    int32_t terminal;    // push of the terminal node
+    int32_t step;        // pop: the step whose right node is emitted
+    double left_weight, right_weight, sd;
+};
+
+// Per-step constants of the LV Log-Euler scheme, all computed on the host in
+// FP64 with the reference's expressions
+// (crates/pricing/src/engine/processes/local_vol.rs:491-513).
+struct LvStep {
+    double t, dt, sqrt_dt, ratio, inv_sqrt_dt, forward, next_forward;
+    int32_t t_cell;
+    int32_t node_mask;   // products observing the step's right node
+    double t_weight;
+};
+
+// Kernel arguments of the BS/Black-76 pathwise kernel.
+struct BsArgs {
+    const ObsDesc* obs;
+    const uint32_t* sobol;     // [scramble][dim][33]: 32 pre-scrambled directions + shift
+    void* partials;            // BlockStat [scramble][block][component]
+    ProductDesc product;
+    Variant variants[kMaxVariants];
+    uint64_t seed;
+    uint64_t units;            // pseudo units or RQMC points per scramble
+    double s0;
+    double expiry;             // T (single-terminal branch)
+    double gamma_bump, spot_bump, vol_bump;
+    uint32_t n_obs;
+    uint32_t n_variants;
+    uint32_t rqmc;
+    uint32_t antithetic;
+    uint32_t risk;
+    uint32_t single_terminal;
+    uint32_t request_gamma;
+    uint32_t zero_variance;
+    uint32_t components;       // statistics written per unit
+    uint32_t sobol_dims;
+};
+
+// Kernel arguments of the local-vol Log-Euler kernel.
+struct LvArgs {
+    const LvStep* steps;
+    const BridgeOp* program;
+    const double* grid;        // [nt][nx] local variance (FP64)
+    const float* grid32;       // FP32 copy for the mixed kernel
+    const double* x_nodes;
+    const float* x_nodes32;
+    const uint32_t* sobol;
+    void* partials;
+    ProductDesc products[kMaxProducts];
+    double weights[kMaxProducts];  // unused; per-node weights live in node_weight
+    const double* node_weight;     // [node][product]
+    uint64_t seed;
+    uint64_t units;
+    double spot;
+    uint32_t n_steps;
+    uint32_t n_program;
+    uint32_t nt, nx;
+    uint32_t n_products;
+    uint32_t rqmc;
+    uint32_t antithetic;
+    uint32_t bridge;
+    uint32_t depth;
+    uint32_t sobol_dims;
+    uint32_t node0_mask;       // products with a fixing at the valuation node
+};
+
+}  // namespace pe

@@ -1,0 +1,121 @@
+"""Python facade over the engine's C ABI, mirroring the reference's
+``PricingPlan.compile(request, ...)`` / ``plan.evaluate()`` surface
+(crates/pricing-python/src/lib.rs:347-395, crates/pricing/src/engine/risk/pricing.rs:53-88).
+
+The HIP library is required: loading fails loudly when it is missing, and
+there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+from . import abi
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "librpl_amd.so")
+_lib = None
+
+BACKENDS = {"fp64": abi.PE_BACKEND_GPU_FP64, "mixed": abi.PE_BACKEND_GPU_MIXED}
+
+
+class EngineError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"pe error {status}: {message}")
+        self.status = status
+
+
+def library_path():
+    return _LIB_PATH
+
+
+def load():
+    """Loads and binds librpl_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise FileNotFoundError(f"{_LIB_PATH} is missing; run __graft_entry__.build() or python -m pricing_amd.build")
+    lib = C.CDLL(_LIB_PATH)
+    abi.bind(lib)
+    if lib.pe_abi_version() != abi.PE_ABI_VERSION:
+        raise RuntimeError("ABI version mismatch")
+    _lib = lib
+    return lib
+
+
+def _check(status):
+    if status != abi.PE_OK:
+        raise EngineError(status, load().pe_last_error().decode())
+
+
+def init(device=0):
+    _check(load().pe_init(device))
+
+
+class PricingPlan:
+    """A compiled plan (host FP64 compile + device tables)."""
+
+    def __init__(self, handle, request):
+        self._handle = C.c_void_p(handle)
+        self._request = request  # keeps the request buffers alive
+
+    @classmethod
+    def compile(cls, request, policy=None):
+        from . import request as _request
+
+        lib = load()
+        pol = policy if policy is not None else _request.exec_policy()
+        out = C.c_void_p()
+        _check(lib.pe_plan_compile(request.pointer(), C.byref(pol), C.byref(out)))
+        return cls(out.value, request)
+
+    def evaluate(self, backend="fp64"):
+        res = abi.pe_result()
+        _check(load().pe_plan_evaluate(self._handle, BACKENDS[backend], C.byref(res)))
+        return res
+
+    def free(self):
+        if self._handle:
+            load().pe_plan_free(self._handle)
+            self._handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def evaluate_shared(plans, backend="fp64"):
+    """Evaluates plans sharing one simulated path set in one launch."""
+    n = len(plans)
+    handles = (C.c_void_p * n)(*[p._handle.value for p in plans])
+    results = (abi.pe_result * n)()
+    _check(load().pe_plans_evaluate_shared(handles, n, BACKENDS[backend], results))
+    return list(results)
+
+
+def local_variance_from_ssvi(theta_times, theta_values, terminal_slope, rho, eta, gamma, time_nodes, x_nodes,
+                             floor, cap):
+    """Dupire local variance of a StandardSsvi surface, built in FP64 on the GPU."""
+    lib = load()
+    s = abi.pe_ssvi_surface()
+    tt = (C.c_double * len(theta_times))(*theta_times)
+    tv = (C.c_double * len(theta_values))(*theta_values)
+    s.theta_count = len(theta_times)
+    s.theta_times = C.cast(tt, C.POINTER(C.c_double))
+    s.theta_values = C.cast(tv, C.POINTER(C.c_double))
+    s.theta_terminal_slope = terminal_slope
+    s.rho, s.eta, s.gamma = rho, eta, gamma
+    t = (C.c_double * len(time_nodes))(*time_nodes)
+    x = (C.c_double * len(x_nodes))(*x_nodes)
+    out = (C.c_double * (len(time_nodes) * len(x_nodes)))()
+    rep = C.c_uint32()
+    _check(lib.pe_local_variance_from_ssvi(C.byref(s), len(time_nodes), t, len(x_nodes), x, floor, cap, out,
+                                           C.byref(rep)))
+    return list(out), rep.value
+
+
+def last_kernel_ms():
+    ms = C.c_double()
+    n = C.c_uint32()
+    load().pe_last_kernel_stats(C.byref(ms), C.byref(n))
+    return ms.value, n.value
