@@ -35,6 +35,18 @@ FP64_PEAK_TFLOPS = 78.6    # MI355X published FP64 vector peak
 FLOPS_PER_PATH_STEP = 73
 
 
+def measured_traffic():
+    """HBM bytes per launch of the mixed kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes, scaled by the same counters on a copy of
+    known size (profiles/lv_kernel_traffic.json)."""
+    path = os.path.join(ROOT, "profiles", "lv_kernel_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def build_plans(points, scrambles):
     expiry = workloads.weekday_dates(252)[-1] / 365.0
     grid = workloads.lv_dupire_grid(expiry, build=engine.local_variance_from_ssvi)
@@ -57,12 +69,13 @@ def time_mode(plans, backend, steps, warmup):
 
 def cpu_baseline(reqs, target_seconds):
     """The CPU FP64 oracle on all host cores on a bounded sample of the same
-    workload (one scramble, a reduced point count)."""
+    workload: the same plans with fewer RQMC points and scrambles, grown until
+    the sample takes about `target_seconds`."""
     from _support import oracle
 
     lib = oracle()
     cores = int(lib.oracle_hardware_threads())
-    points = 1 << 10
+    points, scrambles = 1 << 10, 2
     while True:
         sample = []
         for r in reqs:
@@ -70,7 +83,7 @@ def cpu_baseline(reqs, target_seconds):
             q.c = abi.pe_request.from_buffer_copy(r.c)
             q._keep = r._keep
             q.c.engine.units = points
-            q.c.engine.scrambles = 2
+            q.c.engine.scrambles = scrambles
             sample.append(q)
         arr = (C.POINTER(abi.pe_request) * len(sample))(*[q.pointer() for q in sample])
         out = (abi.pe_result * len(sample))()
@@ -80,14 +93,35 @@ def cpu_baseline(reqs, target_seconds):
         dt = time.perf_counter() - t0
         if rc != 0:
             raise RuntimeError(lib.oracle_last_error().decode())
-        if dt >= target_seconds or points >= 1 << 16:
+        if dt >= 0.5 * target_seconds or scrambles >= 16:
             break
-        points = min(1 << 16, points * max(2, int(target_seconds / max(dt, 1e-3))))
-        points = 1 << (points.bit_length() - 1)
+        grow = max(2, min(16, int(target_seconds / max(dt, 1e-3))))
+        if points < 1 << 16:
+            points = min(1 << 16, points * grow)
+            points = 1 << (points.bit_length() - 1)
+        else:
+            scrambles = min(16, scrambles * grow)
     path_steps = out[0].path_steps
     return {"value": path_steps / dt, "unit": "path-steps/s", "cores": cores, "kind": "port",
-            "sample": f"{points} points x 2 scrambles x {out[0].time_steps} steps ({path_steps} path-steps), "
+            "sample": f"{points} points x {scrambles} scrambles x {out[0].time_steps} steps ({path_steps} path-steps), "
                       f"barrier + Asian on shared paths, FP64 oracle, {dt:.2f} s wall"}
+
+
+def copy_probe(nbytes):
+    """A plain device-to-device copy of known size: the HBM counters it reads
+    (FETCH_SIZE, WRITE_SIZE) calibrate the per-dispatch traffic figures."""
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_probe_bytes": nbytes}))
+    return 0
 
 
 def main():
@@ -100,7 +134,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build", file=sys.stderr)
         return 2
@@ -142,7 +180,7 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic(),
             "kernel": "lv_kernel<float>",
             "avg_kernel_ms": avg_kernel_ms,
             "flops_per_path_step": FLOPS_PER_PATH_STEP,

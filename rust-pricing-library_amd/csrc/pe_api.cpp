@@ -262,8 +262,16 @@ void upload_lv_tables(HostPlan& p) {
 }
 
 // Evaluates local-vol plans that share one simulation grid in one launch.
-void evaluate_lv(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* outs) {
+LvLaunch& prepare_lv(std::vector<HostPlan*>& plans, pe_backend backend) {
     HostPlan& p0 = *plans[0];
+    std::vector<uint64_t> key;
+    for (const HostPlan* q : plans) key.push_back(q->id);
+    if (p0.lv_launch && p0.lv_launch->key == key && p0.lv_launch->backend == static_cast<int>(backend))
+        return *p0.lv_launch;
+    std::unique_ptr<LvLaunch> launch(new LvLaunch());
+    LvLaunch& L = *launch;
+    L.key = key;
+    L.backend = static_cast<int>(backend);
     const pe_engine& e = p0.req.engine;
     if (plans.size() > static_cast<size_t>(kMaxProducts)) throw PlanError(PE_ERR_SIZE_LIMIT, "too many shared plans");
     for (HostPlan* q : plans) {
@@ -279,7 +287,7 @@ void evaluate_lv(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* o
     // Per-node fixing weights and the steps' node masks over all products.
     std::vector<double> node_weight(nodes * kMaxProducts, 0.0);
     std::vector<LvStep> steps = p0.steps;
-    LvArgs a;
+    LvArgs& a = L.args;
     std::memset(&a, 0, sizeof(a));
     for (size_t k = 0; k < plans.size(); ++k) {
         const HostPlan& q = *plans[k];
@@ -293,9 +301,109 @@ void evaluate_lv(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* o
             else a.node0_mask |= 1u << k;
         }
     }
-    DeviceBuffer d_steps, d_node_weight;
+    // Fused instruction stream: bridge pushes and pops, or direct steps, in
+    // an FP32 and an FP64 flavour.
+    std::vector<LvOp32> ops32;
+    std::vector<LvOp64> ops64;
+    auto step_op = [&](uint32_t kind, int32_t step) {
+        const LvStep& st = steps[static_cast<size_t>(step)];
+        LvOp32 o32;
+        LvOp64 o64;
+        std::memset(&o32, 0, sizeof(o32));
+        std::memset(&o64, 0, sizeof(o64));
+        o32.code = o64.code = lv_code(kind, 0u, static_cast<uint32_t>(st.node_mask), static_cast<uint32_t>(st.t_cell));
+        o32.index = o64.index = step;
+        o32.inv_sqrt_dt = st.f_inv_sqrt_dt;
+        o32.m_half_dt = st.m_half_dt;
+        o32.sqrt_dt = st.f_sqrt_dt;
+        o32.t_weight = st.f_t_weight;
+        o32.next_forward = st.f_next_forward;
+        o64.inv_sqrt_dt = st.inv_sqrt_dt;
+        o64.dt = st.dt;
+        o64.sqrt_dt = st.sqrt_dt;
+        o64.ratio = st.ratio;
+        o64.forward = st.forward;
+        o64.t_weight = st.t_weight;
+        ops32.push_back(o32);
+        ops64.push_back(o64);
+    };
+    if (e.brownian_bridge) {
+        for (const BridgeOp& b : p0.program) {
+            if (b.push) {
+                LvOp32 o32;
+                LvOp64 o64;
+                std::memset(&o32, 0, sizeof(o32));
+                std::memset(&o64, 0, sizeof(o64));
+                o32.code = o64.code = lv_code(kLvPush, b.terminal ? 1u : 0u, 0u, 0u);
+                o32.index = o64.index = b.rank;
+                o32.lw = b.f_left_weight;
+                o32.rw = b.f_right_weight;
+                o32.sd = b.f_sd;
+                o64.lw = b.left_weight;
+                o64.rw = b.right_weight;
+                o64.sd = b.sd;
+                ops32.push_back(o32);
+                ops64.push_back(o64);
+            } else {
+                step_op(kLvPop, b.step);
+            }
+        }
+    } else {
+        for (size_t k = 0; k < steps.size(); ++k) step_op(kLvDirect, static_cast<int32_t>(k));
+    }
+    for (size_t k = 0; k < plans.size(); ++k) {
+        const ProductDesc& d = plans[k]->product;
+        ProductHot& h = a.hot[k];
+        h.terminal_node = d.terminal_obs;
+        h.barrier = static_cast<float>(d.barrier);
+        h.half_width = static_cast<float>(d.down ? -d.half_width : d.half_width);
+        switch (d.kind) {
+            case kAsian: h.mode = kModeAsian; break;
+            case kBarrier: h.mode = d.smoothing ? kModeBarrierSmooth : (d.down ? kModeBarrierDown : kModeBarrierUp); break;
+            case kLookback: h.mode = d.call ? kModeLookbackMax : kModeLookbackMin; break;
+            default: h.mode = kModeTerminal;
+        }
+    }
+    // Uniform-bin cell table for the mixed kernel: the smallest power-of-two
+    // bin count (up to 4096) with at most one interior node per bin.
+    std::vector<uint16_t> bin_cell;
+    {
+        const std::vector<double>& xs = p0.lv_x;
+        const double x0 = xs.front(), xl = xs.back();
+        for (uint32_t m = 64; m <= 4096 && bin_cell.empty(); m *= 2) {
+            const double w = (xl - x0) / m;
+            bool ok = true;
+            std::vector<uint16_t> t(m);
+            for (uint32_t b = 0; b < m && ok; ++b) {
+                const double lo = x0 + w * b, hi = x0 + w * (b + 1);
+                uint32_t inside = 0;
+                for (size_t i = 0; i < xs.size(); ++i) inside += (xs[i] > lo && xs[i] < hi) ? 1u : 0u;
+                ok = inside <= 1;
+                size_t c = 0;
+                while (c + 1 < xs.size() - 1 && xs[c + 1] <= lo) ++c;
+                t[b] = static_cast<uint16_t>(c);
+            }
+            if (ok) bin_cell.swap(t);
+        }
+        if (bin_cell.empty()) throw PlanError(PE_ERR_SIZE_LIMIT, "log-moneyness nodes too uneven for the cell table");
+        a.bins = static_cast<uint32_t>(bin_cell.size());
+        a.inv_bin_w32 = static_cast<float>(a.bins / (xl - x0));
+        a.x0_32 = static_cast<float>(x0);
+        a.xl_32 = static_cast<float>(xl);
+    }
+    DeviceBuffer& d_steps = L.d_steps;
+    DeviceBuffer& d_node_weight = L.d_node_weight;
+    DeviceBuffer& d_ops32 = L.d_ops32;
+    DeviceBuffer& d_ops64 = L.d_ops64;
+    L.d_bins.upload(bin_cell);
+    a.bin_cell = L.d_bins.as<uint16_t>();
     d_steps.upload(steps);
     d_node_weight.upload(node_weight);
+    d_ops32.upload(ops32);
+    d_ops64.upload(ops64);
+    a.ops32 = d_ops32.as<LvOp32>();
+    a.ops64 = d_ops64.as<LvOp64>();
+    a.n_ops = static_cast<uint32_t>(ops32.size());
     a.steps = d_steps.as<LvStep>();
     a.program = p0.d_program.as<BridgeOp>();
     a.grid = p0.d_grid.as<double>();
@@ -319,33 +427,51 @@ void evaluate_lv(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* o
     a.sobol_dims = p0.sobol_dims;
     const dim3 grid = grid_for(e);
     const uint32_t comps = kMaxProducts;
-    DeviceBuffer partials, err;
-    partials.reserve(static_cast<size_t>(grid.x) * grid.y * comps * sizeof(BlockStat));
-    err.reserve(sizeof(int));
-    const int no_error = 0x7fffffff;
-    check(hipMemcpy(err.ptr, &no_error, sizeof(int), hipMemcpyHostToDevice), "error word init");
-    a.partials = partials.ptr;
+    L.grid = grid;
+    L.partial_count = static_cast<size_t>(grid.x) * grid.y * comps;
+    L.partials.reserve(L.partial_count * sizeof(BlockStat));
+    L.err.reserve(sizeof(int));
+    check(hipHostMalloc(&L.host_partials, L.partial_count * sizeof(BlockStat), hipHostMallocDefault), "hipHostMalloc");
+    check(hipHostMalloc(reinterpret_cast<void**>(&L.host_err), sizeof(int), hipHostMallocDefault), "hipHostMalloc");
+    a.partials = L.partials.ptr;
+    // LDS: grid, x nodes and inverse widths (Real), the per-wave Sobol words
+    // (RQMC), the bin table (mixed), then the bridge stack (one slot per lane
+    // per tree level).
     const size_t real = backend == PE_BACKEND_GPU_FP64 ? sizeof(double) : sizeof(float);
-    const size_t grid_bytes = ((real * (a.nt * a.nx + a.nx)) + 15u) & ~static_cast<size_t>(15u);
-    const size_t lds = grid_bytes + static_cast<size_t>(std::max<uint32_t>(a.depth, 1u)) * kBlockThreads * sizeof(double);
-    if (lds > 160u * 1024u) throw PlanError(PE_ERR_SIZE_LIMIT, "local-vol grid and bridge stack exceed LDS");
+    const size_t head = real * (a.nt * a.nx + 2u * a.nx) + (a.rqmc ? 4u * sizeof(uint32_t) * a.sobol_dims : 0u);
+    const size_t bin_bytes = backend == PE_BACKEND_GPU_FP64 ? 0u : ((a.bins * sizeof(uint16_t) + 15u) & ~static_cast<size_t>(15u));
+    L.lds = ((head + 15u) & ~static_cast<size_t>(15u)) + bin_bytes +
+            static_cast<size_t>(std::max<uint32_t>(a.depth, 1u)) * kBlockThreads * real;
+    if (L.lds > 160u * 1024u) throw PlanError(PE_ERR_SIZE_LIMIT, "local-vol grid and bridge stack exceed LDS");
+    p0.lv_launch = std::move(launch);
+    return *p0.lv_launch;
+}
+
+void evaluate_lv(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* outs) {
+    LvLaunch& L = prepare_lv(plans, backend);
+    const int no_error = 0x7fffffff;
+    *L.host_err = no_error;
+    check(hipMemcpyAsync(L.err.ptr, L.host_err, sizeof(int), hipMemcpyHostToDevice, g_stream), "error word init");
     const float ms = timed([&] {
-        check(backend == PE_BACKEND_GPU_FP64 ? launch_lv_f64(a, grid, lds, err.as<int>(), g_stream)
-                                             : launch_lv_f32(a, grid, lds, err.as<int>(), g_stream),
+        check(backend == PE_BACKEND_GPU_FP64 ? launch_lv_f64(L.args, L.grid, L.lds, L.err.as<int>(), g_stream)
+                                             : launch_lv_f32(L.args, L.grid, L.lds, L.err.as<int>(), g_stream),
               "lv kernel launch");
     });
-    int failing = 0;
-    check(hipMemcpy(&failing, err.ptr, sizeof(int), hipMemcpyDeviceToHost), "error word copy");
+    check(hipMemcpyAsync(L.host_partials, L.partials.ptr, L.partial_count * sizeof(BlockStat), hipMemcpyDeviceToHost,
+                         g_stream), "partials copy");
+    check(hipMemcpyAsync(L.host_err, L.err.ptr, sizeof(int), hipMemcpyDeviceToHost, g_stream), "error word copy");
+    check(hipStreamSynchronize(g_stream), "stream synchronize");
+    const int failing = *L.host_err;
     if (failing != no_error) {
         for (size_t k = 0; k < plans.size(); ++k) outs[k].failing_unit = static_cast<uint64_t>(failing);
         throw PlanError(PE_ERR_NONFINITE_STATE,
                         "local-vol Log-Euler produced a non-finite state at sampling unit " + std::to_string(failing));
     }
-    std::vector<BlockStat> host(static_cast<size_t>(grid.x) * grid.y * comps);
-    check(hipMemcpy(host.data(), partials.ptr, host.size() * sizeof(BlockStat), hipMemcpyDeviceToHost), "partials copy");
+    const BlockStat* host = static_cast<const BlockStat*>(L.host_partials);
+    const std::vector<BlockStat> partials(host, host + L.partial_count);
     for (size_t k = 0; k < plans.size(); ++k) {
         std::memset(&outs[k], 0, sizeof(pe_result));
-        assemble(*plans[k], host, grid.x, grid.y, comps, static_cast<uint32_t>(k), 1, &outs[k]);
+        assemble(*plans[k], partials, L.grid.x, L.grid.y, kMaxProducts, static_cast<uint32_t>(k), 1, &outs[k]);
         outs[k].device_ms = ms;
     }
     g_kernel_ms = ms;

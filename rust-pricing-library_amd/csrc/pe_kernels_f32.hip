@@ -12,8 +12,7 @@ hipError_t launch_bs_f32(const BsArgs& a, bool risk, dim3 grid, hipStream_t s) {
 }
 
 hipError_t launch_lv_f32(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s) {
-    hipLaunchKernelGGL((lv_kernel<float>), grid, dim3(kBlockThreads), lds, s, a, err);
-    return hipGetLastError();
+    return launch_lv<float>(a, grid, lds, err, s);
 }
 
 }  // namespace pe

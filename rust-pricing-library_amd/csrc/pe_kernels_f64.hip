@@ -12,8 +12,7 @@ hipError_t launch_bs_f64(const BsArgs& a, bool risk, dim3 grid, hipStream_t s) {
 }
 
 hipError_t launch_lv_f64(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s) {
-    hipLaunchKernelGGL((lv_kernel<double>), grid, dim3(kBlockThreads), lds, s, a, err);
-    return hipGetLastError();
+    return launch_lv<double>(a, grid, lds, err, s);
 }
 
 }  // namespace pe

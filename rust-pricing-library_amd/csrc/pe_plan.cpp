@@ -10,6 +10,7 @@
 #include "pe_plan.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -167,21 +168,25 @@ void compile_bridge_program(const std::vector<double>& t, std::vector<BridgeOp>&
     }
     // In-order traversal of the interval tree: values come out in time order.
     program.clear();
-    program.push_back(BridgeOp{1, 0, 1, 0, 0.0, 0.0, std::sqrt(t[n])});
+    auto op = [](int32_t push, int32_t rank, int32_t terminal, int32_t step, double lw, double rw, double sd) {
+        return BridgeOp{push, rank, terminal, step, lw, rw, sd, static_cast<float>(lw), static_cast<float>(rw),
+                        static_cast<float>(sd), 0.0f};
+    };
+    program.push_back(op(1, 0, 1, 0, 0.0, 0.0, std::sqrt(t[n])));
     std::vector<size_t> stack{n};
     depth = 1;
     size_t left = 0;
     while (!stack.empty()) {
         const size_t right = stack.back();
         if (right == left + 1) {
-            program.push_back(BridgeOp{0, 0, 0, static_cast<int32_t>(left), 0.0, 0.0, 0.0});
+            program.push_back(op(0, 0, 0, static_cast<int32_t>(left), 0.0, 0.0, 0.0));
             stack.pop_back();
             left = right;
         } else {
             const auto it = splits.find({left, right});
             if (it == splits.end()) invalid("Brownian bridge interval without a split");
             const Split& s = it->second;
-            program.push_back(BridgeOp{1, static_cast<int32_t>(s.rank), 0, 0, s.lw, s.rw, s.sd});
+            program.push_back(op(1, static_cast<int32_t>(s.rank), 0, 0, s.lw, s.rw, s.sd));
             stack.push_back(s.node);
             depth = std::max<uint32_t>(depth, static_cast<uint32_t>(stack.size()));
         }
@@ -226,6 +231,8 @@ std::vector<uint32_t> build_sobol_table(uint64_t seed, uint32_t scrambles, uint3
 
 // ------------------------------------------------------------------ plan --
 void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* policy) {
+    static std::atomic<uint64_t> next_id{1};
+    p.id = next_id.fetch_add(1);
     p.req = r;
     const pe_product& src = r.product;
     if (src.date_count) {
@@ -275,6 +282,8 @@ void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* p
     d.smoothing = r.risk.smoothing ? 1 : 0;
     d.half_width = r.risk.smoothing_half_width;
     if (d.smoothing && !(d.half_width > 0.0)) invalid("smoothing half width must be positive");
+    d.f_strike = static_cast<float>(d.strike);
+    d.f_half_width = static_cast<float>(d.half_width);
     d.terminal_obs = -1;
     p.obs_weight.assign(p.obs_days.size(), 0.0);
     switch (pr.kind) {
@@ -312,6 +321,7 @@ void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* p
         case PE_PRODUCT_BARRIER:
             d.kind = kBarrier;
             d.barrier = pr.barrier;
+            d.f_barrier = static_cast<float>(pr.barrier);
             d.down = pr.direction_down ? 1 : 0;
             d.knock_in = pr.knock_in ? 1 : 0;
             d.has_rebate = pr.has_rebate ? 1 : 0;
@@ -404,6 +414,12 @@ void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* p
             st.t_cell = static_cast<int32_t>(tc);
             st.t_weight = (st.t - p.lv_t[tc]) / (p.lv_t[tc + 1] - p.lv_t[tc]);
             st.node_mask = 0;
+            st.m_half_dt = static_cast<float>(-0.5 * st.dt);
+            st.f_sqrt_dt = static_cast<float>(st.sqrt_dt);
+            st.f_t_weight = static_cast<float>(st.t_weight);
+            st.f_inv_sqrt_dt = static_cast<float>(st.inv_sqrt_dt);
+            st.f_next_forward = static_cast<float>(st.next_forward);
+            st.pad32[0] = st.pad32[1] = st.pad32[2] = 0.0f;
         }
         // node-indexed terminal for the LV kernel
         if (d.terminal_obs >= 0) d.terminal_obs = static_cast<int32_t>(p.obs_nodes[d.terminal_obs]);

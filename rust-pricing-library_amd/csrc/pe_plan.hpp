@@ -60,7 +60,27 @@ struct HostCurve {
     double log_discount(double t) const;
 };
 
+// Launch state of a local-vol evaluation over one set of plans, cached on the
+// set's first plan so repeated evaluations only launch and read back.
+struct LvLaunch {
+    std::vector<uint64_t> key;       // ids of the plans, in order
+    int backend = 0;
+    LvArgs args{};
+    dim3 grid;
+    size_t lds = 0;
+    DeviceBuffer d_steps, d_node_weight, d_ops32, d_ops64, d_bins, partials, err;
+    void* host_partials = nullptr;   // pinned
+    int* host_err = nullptr;         // pinned
+    size_t partial_count = 0;
+    ~LvLaunch() {
+        if (host_partials) (void)hipHostFree(host_partials);
+        if (host_err) (void)hipHostFree(host_err);
+    }
+};
+
 struct HostPlan {
+    uint64_t id = 0;                 // unique per compiled plan, never reused
+    std::unique_ptr<LvLaunch> lv_launch;
     pe_request req{};
     std::vector<int32_t> dates;
     std::vector<double> weights, fixings;

@@ -33,6 +33,7 @@ struct ProductDesc {
     double discount;         // DF(payment)
     double hist_ext;         // lookback historical extremum
     double asian_prefix;     // Asian: known-fixing prefix sum
+    float f_barrier, f_half_width, f_strike, f_pad;  // FP32 copies for the mixed kernels
 };
 
 // One observation (BS kernel) or one grid node (LV kernel).  mask bit p: the
@@ -62,6 +63,7 @@ struct BridgeOp {
     int32_t terminal;    // push of the terminal node
     int32_t step;        // pop: the step whose right node is emitted
     double left_weight, right_weight, sd;
+    float f_left_weight, f_right_weight, f_sd, pad;
 };
 
 // Per-step constants of the LV Log-Euler scheme, all computed on the host in
@@ -72,6 +74,58 @@ struct LvStep {
     int32_t t_cell;
     int32_t node_mask;   // products observing the step's right node
     double t_weight;
+    // FP32 copies for the mixed kernel (no per-step conversions on the device)
+    float m_half_dt;     // -dt / 2
+    float f_sqrt_dt;
+    float f_t_weight;
+    float f_inv_sqrt_dt;
+    float f_next_forward;
+    float pad32[3];
+};
+
+// One instruction of the local-vol program: a bridge push, a bridge pop that
+// emits step `index`, or (no bridge) a direct step whose shock is normal
+// `index`.  Every constant the step needs travels with it, so one prefetched
+// scalar load per instruction feeds the loop.  code packs
+//   bits 0-1 kind, bit 2 terminal push, bits 4-7 product mask of the emitted
+//   node, bits 8-31 time cell of the local-variance grid.
+enum LvOpKind : uint32_t { kLvPush = 0, kLvPop = 1, kLvDirect = 2 };
+
+__host__ __device__ inline uint32_t lv_code(uint32_t kind, uint32_t terminal, uint32_t mask, uint32_t t_cell) {
+    return kind | (terminal << 2) | (mask << 4) | (t_cell << 8);
+}
+
+struct LvOp32 {          // mixed kernel (12 dwords)
+    uint32_t code;
+    int32_t index;
+    float lw, rw, sd, inv_sqrt_dt;
+    float m_half_dt, sqrt_dt, t_weight, next_forward;
+    float pad0, pad1;
+};
+
+struct LvOp64 {          // FP64 kernel (the reference's constants, bit for bit)
+    uint32_t code;
+    int32_t index;
+    double lw, rw, sd, inv_sqrt_dt;
+    double dt, sqrt_dt, ratio, forward, t_weight;
+};
+
+// Hot per-product state of the local-vol kernel: what a fixing does.
+enum LvMode : int32_t {
+    kModeTerminal = 0,     // European / digital: expiry spot only
+    kModeAsian = 1,        // acc += S * w
+    kModeBarrierUp = 2,    // hard indicator, hit is absorbing
+    kModeBarrierDown = 3,
+    kModeBarrierSmooth = 4,
+    kModeLookbackMax = 5,
+    kModeLookbackMin = 6,
+};
+
+struct ProductHot {
+    int32_t mode;
+    int32_t terminal_node;
+    float barrier;
+    float half_width;
 };
 
 // Kernel arguments of the BS/Black-76 pathwise kernel.
@@ -102,6 +156,16 @@ struct BsArgs {
 struct LvArgs {
     const LvStep* steps;
     const BridgeOp* program;
+    const LvOp32* ops32;       // fused program (bridge pushes + steps), mixed kernel
+    const LvOp64* ops64;       // same program with FP64 constants
+    uint32_t n_ops;
+    uint32_t pad_ops;
+    ProductHot hot[kMaxProducts];
+    // Uniform bins over [x0, xl]: bin_cell[b] is the grid cell of the bin's
+    // left edge; every bin holds at most one interior node (mixed kernel).
+    const uint16_t* bin_cell;
+    uint32_t bins;
+    float inv_bin_w32, x0_32, xl_32;
     const double* grid;        // [nt][nx] local variance (FP64)
     const float* grid32;       // FP32 copy for the mixed kernel
     const double* x_nodes;
