@@ -58,11 +58,14 @@ typedef struct pe_curve {
     const double* discount_factors;
 } pe_curve;
 
-/* DividendQuote::fixed_cash_and_proportional (crates/pricing/src/market/dividend.rs). */
+/* DividendEvent with DividendQuote::FixedCashAndProportional
+ * (crates/pricing/src/market/dividend.rs:8-95; wire form DividendEventV1,
+ * crates/pricing/src/wire.rs:480-492).  ex_time is an ACT/365F year fraction;
+ * events are strictly ascending; fixed_cash >= 0, 0 <= beta < 1. */
 typedef struct pe_dividend_quote {
-    int32_t ex_day;
-    double cash;
-    double proportional;
+    double ex_time;
+    double fixed_cash;
+    double beta;
 } pe_dividend_quote;
 
 typedef struct pe_market {

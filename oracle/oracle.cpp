@@ -12,6 +12,8 @@
 // Execution:        crates/pricing/src/engine/risk/valuation.rs:19-455
 //                   crates/pricing/src/engine/risk/local_vol.rs:24-420
 // BS path/AAD:      crates/pricing/src/engine/processes/black_scholes.rs:41-56, 297-644, 757-793
+// Continuous barrier: crates/pricing/src/engine/processes/black_scholes.rs:58-293, 648-753
+// Dividends:        crates/pricing/src/market/dividend.rs, crates/pricing/src/market/forward.rs
 // LV path:          crates/pricing/src/engine/processes/local_vol.rs:459-668
 // LV valuation:     crates/pricing/src/engine/processes/local_vol_valuation.rs:161-247
 // Estimates:        crates/pricing/src/engine/risk/report.rs:454-473
@@ -29,6 +31,8 @@
 #include <thread>
 #include <vector>
 
+#include "oracle_barrier.hpp"
+#include "oracle_dividend.hpp"
 #include "oracle_market.hpp"
 #include "oracle_numerics.hpp"
 #include "oracle_payoff.hpp"
@@ -38,6 +42,7 @@
 namespace oracle {
 
 constexpr double kNormal95 = 1.959963984540054;  // plan/simulation.rs:24
+constexpr int32_t kNoDay = INT32_MIN;             // observation point without a contractual date
 constexpr double kValidationRelativeSpotBump = 1.0e-4;  // api/mc_result.rs:288
 constexpr double kValidationVolatilityBump = 1.0e-4;    // api/mc_result.rs:289
 
@@ -53,7 +58,9 @@ struct Plan {
     std::vector<double> lv_t, lv_x, lv_v;
 
     Forward market;
-    std::vector<int32_t> obs_days;
+    Dividends div;                     // empty without discrete dividends
+    std::vector<double> div_t, div_cash, div_beta;
+    std::vector<int32_t> obs_days;     // kNoDay marks a dividend-only point
     std::vector<double> obs_times;
     std::vector<double> obs_forwards;
     double time = 0.0;         // expiry year fraction
@@ -69,6 +76,13 @@ struct Plan {
     bool risk = false;
     Tape tape;
     bool single_terminal = false;
+    std::vector<Coord> obs_coord;      // post-dividend affine coordinate per observation
+    std::vector<Coord> obs_pre;        // pre-dividend coordinate (valid where obs_has_pre)
+    std::vector<char> obs_has_pre;
+    bool smoothing = false;
+    Smooth smooth{1.0};
+    bool continuous = false;
+    ContinuousBarrier cb;
 
     // local vol
     bool local_vol = false;
@@ -108,27 +122,120 @@ static void compile_plan(Plan& p, const pe_request& r, const pe_exec_policy* pol
     }
     const pe_market& m = r.market;
     if (!(m.spot > 0.0) || !std::isfinite(m.spot)) throw std::invalid_argument("spot must be positive");
-    if (m.dividend_count != 0) throw std::invalid_argument("discrete dividends are not supported by this oracle path");
     p.market.spot = m.spot;
     p.market.discount_curve.init(m.discount_curve.times, m.discount_curve.discount_factors, m.discount_curve.count);
     p.market.dividend_curve.init(m.dividend_curve.times, m.dividend_curve.discount_factors, m.dividend_curve.count);
     p.spot = m.spot;
+    if (m.dividend_count) {
+        if (r.model.kind == PE_MODEL_LOCAL_VOL)
+            throw std::invalid_argument("discrete dividends under local volatility are not supported by this oracle path");
+        for (uint32_t i = 0; i < m.dividend_count; ++i) {
+            p.div_t.push_back(m.dividends[i].ex_time);
+            p.div_cash.push_back(m.dividends[i].fixed_cash);
+            p.div_beta.push_back(m.dividends[i].beta);
+        }
+        p.div.init(p.spot, p.div_t.data(), p.div_cash.data(), p.div_beta.data(), p.div_t.size(),
+                   &p.market.discount_curve, &p.market.dividend_curve);
+    }
+    const std::vector<Dividends::Entry> timeline = p.div.timeline();
 
     const pe_product& pr = p.req.product;
-    if (pr.kind == PE_PRODUCT_BARRIER && pr.continuous)
-        throw std::invalid_argument("continuous barriers are not supported by this oracle path");
-    p.obs_days = observation_days(pr);
-    ObsIndex idx{p.obs_days};
+    p.continuous = pr.kind == PE_PRODUCT_BARRIER && pr.continuous;
+    if (p.continuous && r.model.kind == PE_MODEL_LOCAL_VOL)
+        throw std::invalid_argument("continuous barriers under local volatility are not supported by this oracle path");
+    const std::vector<int32_t> contractual = observation_days(pr);
+    // Observation points (compile/simulation.rs:210-265): contractual dates,
+    // and for continuous barriers the dividend ex-times up to the monitoring end.
+    std::vector<std::pair<double, int32_t>> points;
+    for (int32_t d : contractual) points.push_back({static_cast<double>(d) / 365.0, d});
+    double monitoring_end = 0.0;
+    if (p.continuous) {
+        if (pr.date_count == 0) throw std::invalid_argument("barrier needs monitoring dates");
+        monitoring_end = static_cast<double>(pr.dates[pr.date_count - 1]) / 365.0;
+        std::vector<std::pair<double, int32_t>> kept;
+        for (const auto& pt : points)
+            if (pt.first > 0.0 || pt.second == pr.expiry_day) kept.push_back(pt);
+        points.swap(kept);
+        for (const auto& e : timeline)
+            if (e.ex_time > 0.0 && e.ex_time <= monitoring_end) points.push_back({e.ex_time, kNoDay});
+        std::stable_sort(points.begin(), points.end(),
+                         [](const auto& a, const auto& b) { return a.first < b.first; });
+        std::vector<std::pair<double, int32_t>> unique;
+        for (const auto& pt : points)
+            if (unique.empty() || unique.back().first != pt.first) unique.push_back(pt);
+        points.swap(unique);
+    }
+    p.obs_days.clear();
     p.obs_times.clear();
-    for (int32_t d : p.obs_days) p.obs_times.push_back(static_cast<double>(d) / 365.0);
-    p.time = p.obs_days.empty() ? 0.0 : static_cast<double>(pr.expiry_day) / 365.0;
+    for (const auto& pt : points) {
+        p.obs_times.push_back(pt.first);
+        p.obs_days.push_back(pt.second);
+    }
+    ObsIndex idx{p.obs_days};
+    p.time = contractual.empty() ? 0.0 : static_cast<double>(pr.expiry_day) / 365.0;
     const double payment_time = static_cast<double>(pr.payment_day) / 365.0;
     p.forward = p.market.forward(p.time);
     p.discount = p.market.discount_curve.discount(payment_time);
 
+    // Discrete barrier dates that coincide with a dividend ex-time are
+    // monitored before and after the jump (compile/simulation.rs:115-130).
+    std::vector<bool> jump_obs(p.obs_days.size(), false);
+    if (pr.kind == PE_PRODUCT_BARRIER)
+        for (uint32_t i = 0; i < pr.date_count; ++i) {
+            if (pr.dates[i] < 0) continue;
+            const double t = static_cast<double>(pr.dates[i]) / 365.0;
+            for (const auto& e : timeline)
+                if (e.ex_time == t) jump_obs[idx.of(pr.dates[i])] = true;
+        }
+
     const pe_risk& rk = r.risk;
-    const bool smoothing = rk.smoothing != 0;
-    p.tape = build_tape(pr, idx, smoothing, rk.smoothing_half_width, {});
+    p.smoothing = rk.smoothing != 0;
+    p.smooth.h = rk.smoothing_half_width;
+    // A continuous barrier is priced by its bridge runtime, not by the graph.
+    if (!p.continuous) p.tape = build_tape(pr, idx, p.smoothing, rk.smoothing_half_width, jump_obs);
+
+    // Affine coordinates per observation (compile/simulation.rs:308-338).
+    p.obs_coord.clear();
+    p.obs_pre.clear();
+    p.obs_has_pre.clear();
+    for (size_t i = 0; i < p.obs_times.size(); ++i) {
+        const ForwardEval fe = evaluate_forward(p.market, &p.div, p.obs_times[i]);
+        p.obs_coord.push_back(fe.coord);
+        const Dividends::Entry* entry = nullptr;
+        for (const auto& e : timeline)
+            if (e.ex_time == p.obs_times[i]) entry = &e;
+        const bool has_pre = entry && (p.continuous || jump_obs[i]);
+        p.obs_has_pre.push_back(has_pre ? 1 : 0);
+        p.obs_pre.push_back(has_pre ? entry->before : Coord{});
+    }
+    if (p.continuous && p.smoothing && !pr.direction_down && pr.historical_hit != 1 &&
+        pr.dates[pr.date_count - 1] >= 0) {
+        // compile/simulation.rs:339-368: the smoothing band must stay below the
+        // transformed spot barrier for every coordinate.
+        std::vector<Coord> coords{Coord{}};
+        coords.insert(coords.end(), p.obs_coord.begin(), p.obs_coord.end());
+        for (size_t i = 0; i < p.obs_pre.size(); ++i)
+            if (p.obs_has_pre[i]) coords.push_back(p.obs_pre[i]);
+        for (const Coord& c : coords)
+            if (rk.smoothing_half_width >= pr.barrier - c.a * p.spot)
+                throw std::invalid_argument("smoothed barrier safe distance exceeds the transformed spot barrier");
+    }
+    if (p.continuous) {
+        ContinuousBarrier& b = p.cb;
+        b.strike = pr.strike;
+        b.barrier = pr.barrier;
+        b.notional = pr.notional;
+        b.rebate = pr.has_rebate ? pr.rebate : 0.0;
+        b.call = pr.side == PE_CALL;
+        b.up = !pr.direction_down;
+        b.knock_in = pr.knock_in != 0;
+        b.monitoring_end = monitoring_end;
+        b.historical_hit = pr.historical_hit;
+        b.bridge_obs.clear();
+        for (size_t i = 0; i < p.obs_times.size(); ++i)
+            if (p.obs_times[i] <= monitoring_end) b.bridge_obs.push_back(i);
+        b.expiry_obs = static_cast<size_t>(idx.of(pr.expiry_day));
+    }
 
     if (r.model.kind == PE_MODEL_BLACK_SCHOLES || r.model.kind == PE_MODEL_BLACK76) {
         p.volatility = r.model.volatility;
@@ -166,24 +273,36 @@ static void compile_plan(Plan& p, const pe_request& r, const pe_exec_policy* pol
     p.request_gamma = rk.gamma != 0;
     p.risk = rk.delta || rk.vega || p.request_gamma;
     if (p.local_vol && p.risk) throw std::invalid_argument("local-vol risk is not supported by this oracle path");
-    if (p.request_gamma) p.gamma_bump = rk.gamma_relative ? p.spot * rk.gamma_bump : rk.gamma_bump;
-    const double residual_spot = p.spot;  // no discrete dividends
+    const double residual_spot = p.spot - p.div.reserve;
+    if (p.request_gamma) {
+        p.gamma_bump = rk.gamma_relative ? p.spot * rk.gamma_bump : rk.gamma_bump;
+        if (!std::isfinite(p.gamma_bump) || p.gamma_bump >= residual_spot) throw std::invalid_argument("invalid gamma bump");
+    }
     p.validation_spot_bump = p.request_gamma ? p.gamma_bump
                                              : std::min(p.spot * kValidationRelativeSpotBump, 0.5 * residual_spot);
     p.validation_vol_bump = p.volatility == 0.0 ? 0.0 : std::min(kValidationVolatilityBump, p.volatility * 0.5);
-    p.single_terminal = p.obs_days.size() == 1 && p.obs_days[0] == pr.expiry_day;
+    p.single_terminal = p.obs_days.size() == 1 && p.obs_days[0] == pr.expiry_day && !p.obs_has_pre[0];
 }
 
 // ------------------------------------------------------------- BS paths --
 struct Obs {
-    double post, f, w;
+    double post, pre, f, w;
 };
 
-// path_observations_from_normals (black_scholes.rs:297-331); identity affine
-// coordinates (no discrete dividends) so post = 0*spot + 1*f.
+// path_observations_from_normals + path_observation (black_scholes.rs:297-354):
+// S = A*spot + B*F with the observation's affine coordinate refitted at `spot`.
 static void bs_observations(const Plan& p, const double* z, double spot, double vol, std::vector<Obs>& out) {
     const double spot_scale = spot / p.spot;
     out.resize(p.obs_times.size());
+    auto finish = [&](size_t i, double f, double w) {
+        const Coord c = p.obs_coord[i].at_spot(p.spot, spot);
+        Obs o{c.a * spot + c.b * f, 0.0, f, w};
+        if (p.obs_has_pre[i]) {
+            const Coord q = p.obs_pre[i].at_spot(p.spot, spot);
+            o.pre = q.a * spot + q.b * f;
+        }
+        out[i] = o;
+    };
     if (p.obs_times.size() == 1) {
         const double t = p.obs_times[0];
         const double tv = vol * vol * t;
@@ -191,7 +310,7 @@ static void bs_observations(const Plan& p, const double* z, double spot, double 
         const double w = std::sqrt(t) * z[0];
         const double lr = -0.5 * tv + sd * z[0];
         const double f = p.obs_forwards[0] * spot_scale * std::exp(lr);
-        out[0] = {0.0 * spot + 1.0 * f, f, w};
+        finish(0, f, w);
         return;
     }
     double prev = 0.0, w = 0.0;
@@ -202,12 +321,16 @@ static void bs_observations(const Plan& p, const double* z, double spot, double 
         prev = t;
         const double tv = vol * vol * t;
         const double f = p.obs_forwards[i] * spot_scale * std::exp(-0.5 * tv + vol * w);
-        out[i] = {0.0 * spot + 1.0 * f, f, w};
+        finish(i, f, w);
     }
 }
 
+constexpr int kBarrierDiagnostics = 8;  // components BARRIER_ENDPOINT_HIT .. BARRIER_CERTAIN_SURVIVAL_COUNT
+
 struct Aad {
     double price, delta, vega;
+    bool has_diag = false;
+    double diag[kBarrierDiagnostics] = {0};
 };
 
 static thread_local std::vector<double> tl_values;
@@ -220,33 +343,242 @@ static Aad aad_single_terminal(const Plan& p, double z, double spot, double vol)
     const double lr = -0.5 * tv + sd * z;
     const double bumped_forward = p.forward * (spot / p.spot);
     const double f = bumped_forward * std::exp(lr);
-    const double terminal = 0.0 * spot + 1.0 * f;  // identity coordinate a=0, b=1
+    const Coord c = p.obs_coord[0].at_spot(p.spot, spot);
+    const double terminal = c.a * spot + c.b * f;
     const double post[1] = {terminal};
     p.tape.forward(post, post, tl_values);
     double post_adj[1] = {0.0}, pre_adj[1] = {0.0};
     p.tape.reverse(tl_values, post_adj, pre_adj);
     const double value = tl_values[p.tape.output];
     const double price = p.discount * value;
-    const double delta = p.discount * post_adj[0] * 1.0 * f / spot;
-    const double terminal_vega = 1.0 * f * (-vol * p.time + std::sqrt(p.time) * z);
+    const double delta = p.discount * post_adj[0] * c.spot_scale * f / spot;
+    const double terminal_vega = c.b * f * (-vol * p.time + std::sqrt(p.time) * z);
     const double vega = p.discount * post_adj[0] * terminal_vega;
     return {price, delta, vega};
 }
 
+// ------------------------------------------------- continuous barriers --
+// ContinuousBarrierBridgeEvaluation (plan/simulation.rs:205-215).
+struct BridgeEval {
+    int kind = 0;  // 0 resolved, 1 exact, 2 smoothed
+    double survival = 1.0;
+    ExactPath path;
+    bool endpoint_touched = false, jump_touched = false;
+    SmoothedPath smoothed;
+    std::vector<std::pair<long, size_t>> interval_obs;  // (left observation or -1, right observation)
+
+    void diagnostics(double* d) const {  // barrier.rs:135-146, 214-231
+        for (int i = 0; i < kBarrierDiagnostics; ++i) d[i] = 0.0;
+        if (kind == 1) {
+            const bool touched = endpoint_touched || jump_touched;
+            d[0] = endpoint_touched ? 1.0 : 0.0;
+            d[1] = jump_touched ? 1.0 : 0.0;
+            d[2] = touched ? 0.0 : 1.0 - path.survival();
+            d[3] = path.counts.interval;
+            d[4] = path.counts.finite;
+            d[5] = path.counts.zero;
+            d[6] = path.counts.underflow;
+            d[7] = path.counts.certain;
+        } else if (kind == 2) {
+            double se = 1.0, sj = 1.0;
+            for (const auto& h : smoothed.hits) {
+                if (h.kind == HitKind::Endpoint) se = se * h.safety;
+                else sj = sj * h.safety;
+            }
+            d[0] = 1.0 - se;
+            d[1] = 1.0 - sj;
+            d[2] = 1.0 - std::exp(smoothed.log_bridge);
+            d[3] = static_cast<double>(smoothed.intervals.size());
+            d[4] = smoothed.counts.finite;
+            d[5] = smoothed.counts.zero;
+            d[6] = smoothed.counts.underflow;
+            d[7] = smoothed.counts.certain;
+        }
+    }
+};
+
+static bool barrier_touched(bool up, double state, double barrier) { return up ? state >= barrier : state <= barrier; }
+
+// continuous_barrier_bridge / smoothed_continuous_barrier_bridge (black_scholes.rs:94-293)
+static void continuous_bridge(const Plan& p, const std::vector<Obs>& obs, double spot, double vol, BridgeEval& e) {
+    const ContinuousBarrier& b = p.cb;
+    e = BridgeEval{};
+    double resolved;
+    if (b.resolved(&resolved)) {
+        e.kind = 0;
+        e.survival = resolved;
+        return;
+    }
+    const Coord init = (p.div.empty() ? Coord{} : p.div.initial()).at_spot(p.spot, spot);
+    const double variance = vol * vol;
+    double prev_state = spot, prev_time = 0.0;
+    long prev_index = -1;
+    if (p.smoothing) {
+        e.kind = 2;
+        const SEndpoint first =
+            smoothed_endpoint(b.up, spot, transformed_barrier(b.barrier, spot, init), init.b, p.smooth);
+        e.smoothed.hits.push_back(endpoint_hit_factor(first, -1));
+        double prev_barrier = transformed_barrier(b.barrier, spot, init);
+        double prev_scale = init.b;
+        for (size_t index : b.bridge_obs) {
+            const double time = p.obs_times[index];
+            const double state = obs[index].f;
+            const Coord post = p.obs_coord[index].at_spot(p.spot, spot);
+            const Coord pre = p.obs_has_pre[index] ? p.obs_pre[index].at_spot(p.spot, spot) : post;
+            const double pre_barrier = transformed_barrier(b.barrier, spot, pre);
+            const double post_barrier = transformed_barrier(b.barrier, spot, post);
+            const double dt = time - prev_time;
+            if (dt > 0.0) {
+                const SInterval it = smoothed_interval(b.up, prev_state, prev_barrier, prev_scale, state, pre_barrier,
+                                                       pre.b, variance, variance, dt, p.smooth);
+                if (p.obs_has_pre[index]) {
+                    const double pre_spot = pre.a * spot + pre.b * state;
+                    const double post_spot = post.a * spot + post.b * state;
+                    e.smoothed.hits.push_back(jump_hit_factor(b.up, pre_spot, post_spot, pre.b, post.b, b.barrier,
+                                                              p.smooth, static_cast<long>(index)));
+                } else {
+                    e.smoothed.hits.push_back(endpoint_hit_factor(it.right, static_cast<long>(index)));
+                }
+                e.smoothed.intervals.push_back(it);
+                e.interval_obs.push_back({prev_index, index});
+            }
+            prev_state = state;
+            prev_barrier = post_barrier;
+            prev_scale = post.b;
+            prev_time = time;
+            prev_index = static_cast<long>(index);
+        }
+        e.smoothed.finish();
+        e.survival = e.smoothed.survival();
+        return;
+    }
+    e.kind = 1;
+    const bool initial_touched = barrier_touched(b.up, spot, b.barrier);
+    double prev_barrier = transformed_barrier(b.barrier, spot, init);
+    std::vector<BridgeInput> inputs;
+    for (size_t index : b.bridge_obs) {
+        if (initial_touched) break;
+        const double time = p.obs_times[index];
+        const double state = obs[index].f;
+        const Coord post = p.obs_coord[index].at_spot(p.spot, spot);
+        const Coord pre = p.obs_has_pre[index] ? p.obs_pre[index].at_spot(p.spot, spot) : post;
+        const double pre_barrier = transformed_barrier(b.barrier, spot, pre);
+        const double post_barrier = transformed_barrier(b.barrier, spot, post);
+        const double dt = time - prev_time;
+        if (dt > 0.0) {
+            inputs.push_back({prev_state, state, prev_barrier, pre_barrier, variance, variance, dt});
+            e.interval_obs.push_back({prev_index, index});
+        }
+        if (p.obs_has_pre[index]) {
+            const double pre_spot = pre.a * spot + pre.b * state;
+            const double post_spot = post.a * spot + post.b * state;
+            const bool pre_t = barrier_touched(b.up, pre_spot, b.barrier);
+            const bool post_t = barrier_touched(b.up, post_spot, b.barrier);
+            if (pre_t || post_t) {
+                e.jump_touched = !pre_t && post_t;
+                break;
+            }
+        }
+        prev_state = state;
+        prev_barrier = post_barrier;
+        prev_time = time;
+        prev_index = static_cast<long>(index);
+    }
+    e.path.evaluate(b.up, inputs);
+    e.endpoint_touched = initial_touched || e.path.counts.touched != 0;
+    e.survival = (e.endpoint_touched || e.jump_touched) ? 0.0 : e.path.survival();
+}
+
+static thread_local BridgeEval tl_bridge;
+
+// continuous_barrier_pathwise_aad (black_scholes.rs:648-753)
+static Aad aad_continuous(const Plan& p, const double* z, double spot, double vol) {
+    bs_observations(p, z, spot, vol, tl_obs);
+    BridgeEval& e = tl_bridge;
+    continuous_bridge(p, tl_obs, spot, vol, e);
+    const ContinuousBarrier& b = p.cb;
+    const double terminal = tl_obs[b.expiry_obs].post;
+    const BarrierTerms terms = continuous_barrier_payoff_terms(b, terminal, e.survival);
+    const size_t n = tl_obs.size();
+    std::vector<double> sa(n, 0.0);
+    const Coord tc = p.obs_coord[b.expiry_obs].at_spot(p.spot, spot);
+    sa[b.expiry_obs] += terms.terminal_derivative * tc.b;
+    double initial_adj = 0.0, variance_adj = 0.0;
+    auto add_state = [&](long index, double v) {
+        if (index >= 0) sa[static_cast<size_t>(index)] += v;
+        else initial_adj += v;
+    };
+    if (e.kind == 1 && !e.endpoint_touched && !e.jump_touched) {
+        const double lsa = terms.survival_derivative * e.survival;
+        if (e.path.counts.touched == 0)
+            for (size_t k = 0; k < e.path.intervals.size(); ++k) {
+                const ExactInterval& it = e.path.intervals[k];
+                add_state(e.interval_obs[k].first, lsa * it.d_left_state);
+                sa[e.interval_obs[k].second] += lsa * it.d_right_state;
+                variance_adj += lsa * it.d_left_var + lsa * it.d_right_var;
+            }
+    } else if (e.kind == 2) {
+        const double lsa = terms.survival_derivative * e.survival;  // SmoothedContinuousBarrierPath::reverse
+        for (size_t k = 0; k < e.smoothed.intervals.size(); ++k) {
+            const SInterval& it = e.smoothed.intervals[k];
+            add_state(e.interval_obs[k].first, it.left.reverse_state(0.0, lsa * it.d_ld));
+            sa[e.interval_obs[k].second] += it.right.reverse_state(0.0, lsa * it.d_rd);
+            variance_adj += lsa * it.d_lv + lsa * it.d_rv;
+        }
+        for (const HitFactor& h : e.smoothed.hits) {
+            const double d = h.safety == 0.0 ? 0.0 : -lsa * h.state_derivative / h.safety;
+            add_state(h.state_index, d);
+        }
+    }
+    double delta = initial_adj;
+    double vega = 2.0 * vol * variance_adj;
+    for (size_t i = 0; i < n; ++i) {
+        delta += sa[i] * tl_obs[i].f / spot;
+        vega += sa[i] * tl_obs[i].f * (-vol * p.obs_times[i] + tl_obs[i].w);
+    }
+    delta = p.discount * delta;
+    if (p.div.reserve != 0.0) delta = delta * spot / (spot - p.div.reserve);
+    Aad out{p.discount * terms.value, delta, p.discount * vega};
+    out.has_diag = true;
+    e.diagnostics(out.diag);
+    return out;
+}
+
+// continuous_barrier_path_values_from_normals (black_scholes.rs:58-78)
+static void continuous_price_values(const Plan& p, const double* z, double* v) {
+    bs_observations(p, z, p.spot, p.volatility, tl_obs);
+    continuous_bridge(p, tl_obs, p.spot, p.volatility, tl_bridge);
+    const BarrierTerms terms =
+        continuous_barrier_payoff_terms(p.cb, tl_obs[p.cb.expiry_obs].post, tl_bridge.survival);
+    v[PE_C_PRICE] = p.discount * terms.value;
+    tl_bridge.diagnostics(v + PE_C_BARRIER_ENDPOINT_HIT);
+}
+
 // pathwise_aad (black_scholes.rs:476-563), generic observation loop.
 static Aad aad_generic(const Plan& p, const double* z, double spot, double vol) {
+    if (p.continuous) return aad_continuous(p, z, spot, vol);
     if (p.single_terminal) return aad_single_terminal(p, z[0], spot, vol);
     bs_observations(p, z, spot, vol, tl_obs);
     const size_t n = tl_obs.size();
-    std::vector<double> post(n), post_adj(n, 0.0), pre_adj(n, 0.0);
-    for (size_t i = 0; i < n; ++i) post[i] = tl_obs[i].post;
-    p.tape.forward(post.data(), post.data(), tl_values);
+    std::vector<double> post(n), pre(n), post_adj(n, 0.0), pre_adj(n, 0.0);
+    for (size_t i = 0; i < n; ++i) {
+        post[i] = tl_obs[i].post;
+        pre[i] = tl_obs[i].pre;
+    }
+    p.tape.forward(post.data(), pre.data(), tl_values);
     p.tape.reverse(tl_values, post_adj.data(), pre_adj.data());
     double delta = 0.0, vega = 0.0;
     for (size_t i = 0; i < n; ++i) {
         if (post_adj[i] == 0.0) continue;
-        delta += post_adj[i] * 1.0 * tl_obs[i].f / spot;
-        vega += post_adj[i] * 1.0 * tl_obs[i].f * (-vol * p.obs_times[i] + tl_obs[i].w);
+        const Coord c = p.obs_coord[i].at_spot(p.spot, spot);
+        delta += post_adj[i] * c.spot_scale * tl_obs[i].f / spot;
+        vega += post_adj[i] * c.b * tl_obs[i].f * (-vol * p.obs_times[i] + tl_obs[i].w);
+    }
+    for (size_t i = 0; i < n; ++i) {
+        if (pre_adj[i] == 0.0 || !p.obs_has_pre[i]) continue;
+        const Coord c = p.obs_pre[i].at_spot(p.spot, spot);
+        delta += pre_adj[i] * c.spot_scale * tl_obs[i].f / spot;
+        vega += pre_adj[i] * c.b * tl_obs[i].f * (-vol * p.obs_times[i] + tl_obs[i].w);
     }
     return {p.discount * tl_values[p.tape.output], p.discount * delta, p.discount * vega};
 }
@@ -254,9 +586,12 @@ static Aad aad_generic(const Plan& p, const double* z, double spot, double vol) 
 // discounted_payoff_from_normals (black_scholes.rs:41-56)
 static double bs_price(const Plan& p, const double* z) {
     bs_observations(p, z, p.spot, p.volatility, tl_obs);
-    std::vector<double> post(tl_obs.size());
-    for (size_t i = 0; i < post.size(); ++i) post[i] = tl_obs[i].post;
-    p.tape.forward(post.data(), post.data(), tl_values);
+    std::vector<double> post(tl_obs.size()), pre(tl_obs.size());
+    for (size_t i = 0; i < post.size(); ++i) {
+        post[i] = tl_obs[i].post;
+        pre[i] = tl_obs[i].pre;
+    }
+    p.tape.forward(post.data(), pre.data(), tl_values);
     return p.discount * tl_values[p.tape.output];
 }
 
@@ -292,6 +627,8 @@ static void bs_pathwise_values(const Plan& p, const double* z, double* v) {
     v[PE_C_VEGA_DIFFERENCE] = bump_vega - base.vega;
     v[PE_C_BUMP_GAMMA] = bump_gamma;
     v[PE_C_GAMMA_DIFFERENCE] = bump_gamma - gamma;
+    if (base.has_diag)
+        for (int i = 0; i < kBarrierDiagnostics; ++i) v[PE_C_BARRIER_ENDPOINT_HIT + i] = base.diag[i];
 }
 
 // ------------------------------------------------------------- LV paths --
@@ -407,6 +744,8 @@ static UnitFn make_unit(const Plan& p, const RqmcPlan* q, const BridgePlan* brid
                 out[PE_C_PRICE] = lv_price(p, zz.data());
             } else if (p.risk) {
                 bs_pathwise_values(p, zz.data(), out);
+            } else if (p.continuous) {
+                continuous_price_values(p, zz.data(), out);
             } else {
                 out[PE_C_PRICE] = bs_price(p, zz.data());
             }
@@ -710,5 +1049,117 @@ int oracle_evaluate_shared(const pe_request* const* reqs, uint32_t count, const 
 }
 
 unsigned oracle_hardware_threads(void) { return std::max(1u, std::thread::hardware_concurrency()); }
+
+}  // extern "C"
+
+extern "C" {
+
+// CompactC2Smoothing (smoothing.rs:28-175): out = indicator value/first/second,
+// positive part value/first/second, maximum, minimum of (x, y).
+void oracle_smoothing(double h, double x, double y, double out[8]) {
+    Smooth sm{h};
+    double v, f, comp;
+    sm.indicator_with_complement(x, &v, &f, &comp);
+    double second = 0.0;
+    if (x > -h && x < h) {
+        double iv, ifirst, icomp;
+        sm.interior(x, sm.transition(x), &iv, &ifirst, &second, &icomp);
+    }
+    out[0] = v; out[1] = f; out[2] = second;
+    double pv, pf, pc;
+    sm.positive_part(x, &pv, &pf, &pc);
+    out[3] = pv; out[4] = pf; out[5] = (x > -h && x < h) ? f : 0.0;
+    double mv, lf, rf;
+    sm.maximum(x, y, &mv, &lf, &rf);
+    out[6] = mv;
+    const double d = x - y;
+    if (d >= h) out[7] = y;
+    else if (d <= -h) out[7] = x;
+    else { sm.positive_part(d, &pv, &pf, &pc); out[7] = x - pv; }
+}
+
+// Exact bridge interval (barrier_bridge.rs:446-595): out = status, log survival,
+// d/d(left state, right state, left barrier, right barrier, left var, right var, dt).
+int oracle_bridge_interval(int up, double ls, double rs, double lb, double rb, double lv, double rv, double dt,
+                           double out[9]) {
+    try {
+        const ExactInterval it = exact_interval(up != 0, ls, rs, lb, rb, lv, rv, dt);
+        out[0] = static_cast<double>(static_cast<int>(it.status));
+        out[1] = it.log_survival;
+        out[2] = it.d_left_state; out[3] = it.d_right_state;
+        out[4] = it.d_left_barrier; out[5] = it.d_right_barrier;
+        out[6] = it.d_left_var; out[7] = it.d_right_var; out[8] = it.d_dt;
+        return 0;
+    } catch (const std::exception& e) { return fail(e, 1); }
+}
+
+// One smoothed interval with identity coordinates, unit length and variance
+// `iv` at both ends, composed with its two endpoint hit factors as the
+// smoothed path does (barrier.rs:57-116): out = start/end hit weight,
+// start/end effective log distance, bridge survival, endpoint survival,
+// total survival, total hit weight, and the interval's log-survival
+// derivatives w.r.t. the two distances and the two variances.
+int oracle_smoothed_bridge(int up, double start, double end, double barrier, double h, double iv, double out[12]) {
+    try {
+        Smooth sm{h};
+        const SInterval it = smoothed_interval(up != 0, start, barrier, 1.0, end, barrier, 1.0, iv, iv, 1.0, sm);
+        SmoothedPath path;
+        path.intervals.push_back(it);
+        path.hits.push_back(endpoint_hit_factor(it.left, -1));
+        path.hits.push_back(endpoint_hit_factor(it.right, 0));
+        path.finish();
+        out[0] = it.left.hit; out[1] = it.right.hit;
+        out[2] = it.left.eld; out[3] = it.right.eld;
+        out[4] = std::exp(path.log_bridge);
+        out[5] = std::exp(path.log_hit);
+        out[6] = path.survival();
+        out[7] = 1.0 - path.survival();
+        out[8] = it.d_ld; out[9] = it.d_rd; out[10] = it.d_lv; out[11] = it.d_rv;
+        return 0;
+    } catch (const std::exception& e) { return fail(e, 1); }
+}
+
+// Smoothed dividend-jump hit factor (barrier.rs:275-304): out = smoothed
+// signed score, hit weight, safety weight, state derivative (unit scales).
+void oracle_jump_hit(int up, double pre, double post, double barrier, double h, double out[4]) {
+    Smooth sm{h};
+    const double sign = up ? 1.0 : -1.0;
+    double score, lf, rf;
+    sm.maximum(sign * (pre - barrier), sign * (post - barrier), &score, &lf, &rf);
+    double hit, first, safety;
+    sm.indicator_with_complement(score, &hit, &first, &safety);
+    const HitFactor f = jump_hit_factor(up != 0, pre, post, 1.0, 1.0, barrier, sm, 0);
+    out[0] = score; out[1] = hit; out[2] = f.safety; out[3] = f.state_derivative;
+}
+
+// One dividend event applied to an affine coordinate (dividend.rs:127-205):
+// out = alpha, A after, B after, post-event spot reconstructed at F = f.
+void oracle_dividend_event(double s0, double a, double b, double cash, double beta, double f, double out[4]) {
+    const double alpha = cash / s0;
+    Coord c;
+    c.a = a;
+    c.b = b;
+    const Coord n = Dividends::after(c, DividendEventC{0.0, cash, alpha, beta});
+    out[0] = alpha; out[1] = n.a; out[2] = n.b; out[3] = n.a * s0 + n.b * f;
+}
+
+// Affine coordinates of a compiled plan's observations: rows of
+// (time, A, B, spot_scale, has_pre, pre A, pre B, pre spot_scale); returns the count.
+int oracle_plan_coordinates(const pe_request* req, double* rows, uint32_t capacity, double* reserve) {
+    try {
+        Plan p;
+        compile_plan(p, *req, nullptr);
+        if (reserve) *reserve = p.div.reserve;
+        const size_t n = p.obs_times.size();
+        for (size_t i = 0; i < n && i < capacity; ++i) {
+            double* r = rows + 8 * i;
+            r[0] = p.obs_times[i];
+            r[1] = p.obs_coord[i].a; r[2] = p.obs_coord[i].b; r[3] = p.obs_coord[i].spot_scale;
+            r[4] = p.obs_has_pre[i];
+            r[5] = p.obs_pre[i].a; r[6] = p.obs_pre[i].b; r[7] = p.obs_pre[i].spot_scale;
+        }
+        return static_cast<int>(n);
+    } catch (const std::exception& e) { return -fail(e, 1); }
+}
 
 }  // extern "C"

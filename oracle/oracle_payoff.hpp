@@ -47,10 +47,16 @@ struct Smooth {  // CompactC2Smoothing
         *comp = c;
     }
     void indicator(double input, double* value, double* first) const {
-        if (input <= -h) { *value = 0.0; *first = 0.0; return; }
-        if (input >= h) { *value = 1.0; *first = 0.0; return; }
-        double second, comp;
-        interior(input, transition(input), value, first, &second, &comp);
+        double comp;
+        indicator_with_complement(input, value, first, &comp);
+    }
+    // indicator_with_complement (smoothing.rs:34-50): the hit weight, its first
+    // derivative and the safety weight 1 - hit computed without cancellation.
+    void indicator_with_complement(double input, double* value, double* first, double* comp) const {
+        if (input <= -h) { *value = 0.0; *first = 0.0; *comp = 1.0; return; }
+        if (input >= h) { *value = 1.0; *first = 0.0; *comp = 0.0; return; }
+        double second;
+        interior(input, transition(input), value, first, &second, comp);
     }
     void positive_part(double input, double* value, double* first, double* comp) const {
         if (input <= -h) { *value = 0.0; *first = 0.0; *comp = 1.0; return; }

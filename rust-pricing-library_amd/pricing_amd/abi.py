@@ -43,6 +43,9 @@ COMPONENT_NAMES = (
     "barrier_finite_correction_count", "barrier_zero_variance_count",
     "barrier_survival_underflow_count", "barrier_certain_survival_count",
 )
+# pe_component indices (PE_C_PRICE = 0 ... PE_C_BARRIER_CERTAIN_SURVIVAL_COUNT = 17)
+for _i, _n in enumerate(COMPONENT_NAMES):
+    globals()["PE_C_" + _n.upper()] = _i
 
 _DP = C.POINTER(C.c_double)
 _IP = C.POINTER(C.c_int32)
@@ -53,7 +56,7 @@ class pe_curve(C.Structure):
 
 
 class pe_dividend_quote(C.Structure):
-    _fields_ = [("ex_day", C.c_int32), ("cash", C.c_double), ("proportional", C.c_double)]
+    _fields_ = [("ex_time", C.c_double), ("fixed_cash", C.c_double), ("beta", C.c_double)]
 
 
 class pe_market(C.Structure):

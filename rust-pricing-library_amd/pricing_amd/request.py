@@ -65,10 +65,10 @@ class Request:
         m.dividend_curve.discount_factors = self._ptr_d(dividend_factors)
         if dividends:
             arr = (abi.pe_dividend_quote * len(dividends))()
-            for i, (day, cash, prop) in enumerate(dividends):
-                arr[i].ex_day = int(day)
-                arr[i].cash = float(cash)
-                arr[i].proportional = float(prop)
+            for i, (ex_time, fixed_cash, beta) in enumerate(dividends):
+                arr[i].ex_time = float(ex_time)
+                arr[i].fixed_cash = float(fixed_cash)
+                arr[i].beta = float(beta)
             self._keep.append(arr)
             m.dividend_count = len(dividends)
             m.dividends = C.cast(arr, C.POINTER(abi.pe_dividend_quote))
@@ -166,7 +166,16 @@ class Request:
             raise UnsupportedRequest("only equity markets are supported")
         dividends = []
         for d in mk.get("discrete_dividends", []) or []:
-            raise UnsupportedRequest("discrete dividends in v3 documents are not supported yet")
+            # DividendEventV1 (crates/pricing/src/wire.rs:480-492, 1617-1633)
+            q = d["quote"]
+            if q["type"] == "fixed_cash":
+                dividends.append((d["ex_time"], q["amount"], 0.0))
+            elif q["type"] == "proportional":
+                dividends.append((d["ex_time"], 0.0, q["beta"]))
+            elif q["type"] == "fixed_cash_and_proportional":
+                dividends.append((d["ex_time"], q["fixed_cash"], q["beta"]))
+            else:
+                raise UnsupportedRequest(f"dividend quote {q['type']}")
         self.set_market(mk["spot"], mk["discount_curve"]["times"], mk["discount_curve"]["discount_factors"],
                         mk["dividend_curve"]["times"], mk["dividend_curve"]["discount_factors"], dividends)
         model = doc["model"]

@@ -61,6 +61,12 @@ def oracle():
     lib.oracle_evaluate_shared.argtypes = [C.POINTER(C.POINTER(abi.pe_request)), C.c_uint32,
                                            C.POINTER(abi.pe_exec_policy), C.POINTER(abi.pe_result)]
     lib.oracle_hardware_threads.restype = C.c_uint
+    lib.oracle_smoothing.argtypes = [C.c_double, C.c_double, C.c_double, D]
+    lib.oracle_bridge_interval.argtypes = [C.c_int] + [C.c_double] * 7 + [D]
+    lib.oracle_smoothed_bridge.argtypes = [C.c_int] + [C.c_double] * 5 + [D]
+    lib.oracle_jump_hit.argtypes = [C.c_int] + [C.c_double] * 4 + [D]
+    lib.oracle_dividend_event.argtypes = [C.c_double] * 6 + [D]
+    lib.oracle_plan_coordinates.argtypes = [C.POINTER(abi.pe_request), D, C.c_uint32, D]
     if lib.oracle_init(directions_path().encode()) != 0:
         raise RuntimeError(lib.oracle_last_error().decode())
     _oracle = lib

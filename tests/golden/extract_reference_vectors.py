@@ -6,7 +6,8 @@ request documents, expected estimates and Decimal reference values copied
 from
 
   fixtures/replay/{european_bs,local_volatility,path_dependence}-windows-x86_64.json
-  fixtures/local-vol/reference-cases-v0.1.json  (standard_ssvi, constant_vol_dupire)
+  fixtures/local-vol/reference-cases-v0.1.json  (standard_ssvi, constant_vol_dupire, affine_dividend)
+  fixtures/path-dependence/reference-cases-v0.1.json  (smoothing, extrema, bridge, smoothed bridge, jumps)
 
 The tests read the JSON this script writes; nothing at test time reads the
 reference tree.
@@ -40,6 +41,8 @@ def replay(name):
             "risks": risks,
             "sampling_variance_bits": case["execution"]["sampling_variance_bits"],
             "evaluated_paths": case["execution"]["evaluated_paths"],
+            "risk_validation": case["execution"].get("risk_validation"),
+            "barrier_bridge": (case.get("path_diagnostics") or {}).get("barrier_bridge"),
         })
     return cases
 
@@ -53,6 +56,11 @@ def main():
         lv = json.load(f)
     out["standard_ssvi"] = [c for c in lv["standard_ssvi"] if "eta" in c["inputs"]]
     out["constant_vol_dupire"] = lv["constant_vol_dupire"]
+    out["affine_dividend"] = lv["affine_dividend"]
+    with open(os.path.join(ROOT, "fixtures", "path-dependence", "reference-cases-v0.1.json")) as f:
+        pd = json.load(f)
+    out["path_dependence"] = {k: pd[k] for k in ("smoothing_cases", "extrema_cases", "bridge_cases",
+                                                "smoothed_bridge_cases", "jump_cases")}
     with open(os.path.join(HERE, "reference_vectors.json"), "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
         f.write("\n")

@@ -9,11 +9,10 @@ from _support import golden, oracle_evaluate
 from pricing_amd import request
 
 UNSUPPORTED = {
-    # local-vol Delta/Gamma/VegaKT risk and continuous barriers are not yet in
-    # the oracle; the price-only and discrete cases pin the same paths.
+    # local-vol Delta/Gamma/VegaKT risk is not yet in the oracle; the
+    # price-only cases pin the same paths.
     "pseudo_mc_delta_gamma_vega_vegakt",
     "rqmc_delta_gamma_vega_vegakt",
-    "continuous_barrier_smoothed_full_risk",
 }
 
 CASES = [c for c in golden()["replay"]]
