@@ -254,6 +254,17 @@ pe_status pe_plan_evaluate(const pe_plan* plan, pe_backend backend, pe_result* o
 pe_status pe_plans_evaluate_shared(const pe_plan* const* plans, uint32_t count,
                                    pe_backend backend, pe_result* out);
 
+/* Evaluates `count` independent Black-Scholes / Black-76 plans (thousands of
+ * trades with uneven step and path counts) in persistent launches that share
+ * the device: results[i] equals what pe_plan_evaluate returns for plans[i].
+ * Each plan follows SimulationPlan::execute
+ * (crates/pricing/src/engine/risk/valuation.rs:19-35); the batch has no
+ * reference counterpart beyond calling price_monte_carlo once per trade
+ * (crates/pricing/src/engine/risk/valuation.rs:440-455).  A path that fails
+ * fails the call; the message names the trade and the sampling unit. */
+pe_status pe_batch_evaluate(const pe_plan* const* plans, uint32_t count, pe_backend backend,
+                            pe_result* results);
+
 /* LocalVarianceGrid::from_surface for a StandardSsvi surface with a ThetaPchip
  * total-variance curve and a power-law phi, computed in FP64 on the device
  * (crates/pricing/src/market/local_variance.rs:149-206, ssvi.rs:61-186,226-275,

@@ -176,78 +176,177 @@ float timed(F&& launch) {
     return ms;
 }
 
-void evaluate_bs(HostPlan& p, pe_backend backend, pe_result* out) {
-    const pe_engine& e = p.req.engine;
-    ensure_sobol(p);
-    if (!p.d_obs.ptr) {
-        std::vector<ObsDesc> obs(p.obs_times.size());
-        double prev = 0.0;
-        for (size_t i = 0; i < obs.size(); ++i) {
-            ObsDesc& o = obs[i];
-            std::memset(&o, 0, sizeof(o));
-            o.time = p.obs_times[i];
-            o.sqrt_step = std::sqrt(std::max(o.time - prev, 0.0));
-            prev = o.time;
-            o.forward = p.obs_forwards[i];
-            o.weight[0] = p.obs_weight[i];
-            o.mask = p.obs_weight[i] != 0.0 ? 1u : 0u;
-        }
-        p.d_obs.upload(obs);
+// ---------------------------------------------------------- BS batches --
+// A batch of BS/Black-76 plans runs as one launch per kernel flavour
+// (price-only / risk x discrete / continuous barrier), one workgroup per work
+// item.  Work items are 4096-unit ranges of one trade and scramble, ordered
+// longest-first; each has a fixed partial slot so the per-trade merge below
+// is in unit order whatever the schedule.
+constexpr uint64_t kItemUnits = 4096;
+
+}  // namespace
+
+struct BsGroup {
+    bool risk = false, cont = false;
+    uint32_t components = 1;
+    std::vector<BsItem> items;
+    size_t partial_base = 0;        // BlockStats before this group's slots
+    DeviceBuffer d_items;
+};
+
+struct BsPlanSlot {
+    int group = 0;
+    uint32_t first_partial = 0;     // within the group
+    uint32_t chunks = 0;
+};
+
+struct BsLaunch {
+    std::vector<uint64_t> key;
+    int backend = 0;
+    std::vector<BsTrade> trades;
+    std::vector<BsPlanSlot> slots;
+    BsGroup groups[4];
+    DeviceBuffer d_trades, d_obs, d_sobol, partials, err;
+    std::vector<BlockStat> host_partials;
+    size_t partial_count = 0;
+};
+
+void BsLaunchDeleter::operator()(BsLaunch* l) const { delete l; }
+
+namespace {
+
+// Launch cache per plan set, owned by the set's first plan.
+std::mutex g_bs_cache_mutex;
+
+BsLaunch& prepare_bs(std::vector<HostPlan*>& plans, pe_backend backend) {
+    HostPlan& p0 = *plans[0];
+    std::vector<uint64_t> key;
+    for (const HostPlan* q : plans) key.push_back(q->id);
+    if (p0.bs_launch && p0.bs_launch->key == key && p0.bs_launch->backend == static_cast<int>(backend))
+        return *p0.bs_launch;
+    auto L = std::make_unique<BsLaunch>();
+    L->key = key;
+    L->backend = static_cast<int>(backend);
+    std::vector<BsObs> obs;
+    std::vector<uint32_t> sobol;
+    for (int g = 0; g < 4; ++g) {
+        L->groups[g].risk = (g & 1) != 0;
+        L->groups[g].cont = (g & 2) != 0;
+        L->groups[g].components = L->groups[g].cont ? PE_COMPONENTS : (L->groups[g].risk ? 10u : 1u);
     }
-    BsArgs a;
-    std::memset(&a, 0, sizeof(a));
-    a.obs = p.d_obs.as<ObsDesc>();
-    a.sobol = p.d_sobol.as<uint32_t>();
-    a.product = p.product;
-    a.seed = e.seed;
-    a.units = e.units;
-    a.s0 = p.spot;
-    a.expiry = p.expiry;
-    a.gamma_bump = p.gamma_bump;
-    a.spot_bump = p.spot_bump;
-    a.vol_bump = p.vol_bump;
-    a.n_obs = static_cast<uint32_t>(p.obs_times.size());
-    a.rqmc = e.kind == PE_ENGINE_RQMC;
-    a.antithetic = e.antithetic ? 1u : 0u;
-    a.risk = p.risk ? 1u : 0u;
-    a.single_terminal = a.n_obs == 1 ? 1u : 0u;
-    a.request_gamma = p.request_gamma ? 1u : 0u;
-    a.zero_variance = p.total_variance == 0.0 ? 1u : 0u;
-    a.sobol_dims = p.sobol_dims;
-    auto variant = [&](int k, double spot, double vol) {
-        a.variants[k] = Variant{spot, vol, spot / p.spot};
-    };
-    variant(0, p.spot, p.vol);
-    uint32_t nv = 1;
-    if (p.risk) {
-        variant(1, p.spot - p.spot_bump, p.vol);
-        variant(2, p.spot + p.spot_bump, p.vol);
-        nv = 3;
-        if (p.vol_bump != 0.0) {
-            variant(3, p.spot, p.vol - p.vol_bump);
-            variant(4, p.spot, p.vol + p.vol_bump);
-            nv = 5;
+    uint32_t group_partials[4] = {0, 0, 0, 0};
+    for (size_t ti = 0; ti < plans.size(); ++ti) {
+        HostPlan& p = *plans[ti];
+        if (p.lv) throw PlanError(PE_ERR_INVALID_ARGUMENT, "local-vol plans are evaluated by pe_plans_evaluate_shared");
+        BsTrade t = p.trade;
+        t.obs_offset = static_cast<uint32_t>(obs.size());
+        obs.insert(obs.end(), p.bs_obs.begin(), p.bs_obs.end());
+        const pe_engine& e = p.req.engine;
+        const uint32_t scrambles = e.kind == PE_ENGINE_RQMC ? e.scrambles : 1u;
+        if (e.kind == PE_ENGINE_RQMC) {
+            t.sobol_offset = static_cast<uint32_t>(sobol.size());
+            const std::vector<uint32_t> tab = build_sobol_table(e.seed, e.scrambles, p.sobol_dims);
+            if (sobol.size() + tab.size() > 0xffffffffull) throw PlanError(PE_ERR_SIZE_LIMIT, "Sobol tables");
+            sobol.insert(sobol.end(), tab.begin(), tab.end());
         }
+        L->trades.push_back(t);
+        const int g = (p.risk ? 1 : 0) | (p.continuous ? 2 : 0);
+        BsGroup& grp = L->groups[g];
+        BsPlanSlot slot;
+        slot.group = g;
+        slot.chunks = static_cast<uint32_t>((e.units + kItemUnits - 1) / kItemUnits);
+        slot.first_partial = group_partials[g];
+        for (uint32_t sc = 0; sc < scrambles; ++sc)
+            for (uint32_t c = 0; c < slot.chunks; ++c) {
+                BsItem it{};
+                it.trade = static_cast<uint32_t>(ti);
+                it.scramble = sc;
+                it.unit_begin = static_cast<uint64_t>(c) * kItemUnits;
+                it.unit_end = std::min<uint64_t>(it.unit_begin + kItemUnits, e.units);
+                it.partial = slot.first_partial + sc * slot.chunks + c;
+                grp.items.push_back(it);
+            }
+        group_partials[g] += scrambles * slot.chunks;
+        L->slots.push_back(slot);
     }
-    a.n_variants = nv;
-    const uint32_t comps = p.risk ? 10u : 1u;
-    a.components = comps;
-    const dim3 grid = grid_for(e);
-    DeviceBuffer partials;
-    partials.reserve(static_cast<size_t>(grid.x) * grid.y * comps * sizeof(BlockStat));
-    a.partials = partials.ptr;
+    size_t base = 0;
+    for (int g = 0; g < 4; ++g) {
+        BsGroup& grp = L->groups[g];
+        grp.partial_base = base;
+        base += static_cast<size_t>(group_partials[g]) * grp.components;
+        // longest items first: the work queue then ends with short items
+        std::stable_sort(grp.items.begin(), grp.items.end(), [&](const BsItem& x, const BsItem& y) {
+            const BsTrade& tx = L->trades[x.trade];
+            const BsTrade& ty = L->trades[y.trade];
+            const uint64_t cx = (x.unit_end - x.unit_begin) * tx.n_obs * tx.n_variants;
+            const uint64_t cy = (y.unit_end - y.unit_begin) * ty.n_obs * ty.n_variants;
+            return cx > cy;
+        });
+        if (grp.items.size() > 0x7fffffffu) throw PlanError(PE_ERR_SIZE_LIMIT, "too many work items");
+        if (!grp.items.empty()) grp.d_items.upload(grp.items);
+    }
+    L->partial_count = base;
+    L->d_trades.upload(L->trades);
+    L->d_obs.upload(obs);
+    if (!sobol.empty()) L->d_sobol.upload(sobol);
+    L->partials.reserve(std::max<size_t>(1, base) * sizeof(BlockStat));
+    L->err.reserve(sizeof(unsigned long long));
+    L->host_partials.resize(base);
+    p0.bs_launch.reset(L.release());
+    return *p0.bs_launch;
+}
+
+void evaluate_bs_batch(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* outs) {
+    BsLaunch& L = prepare_bs(plans, backend);
+    const unsigned long long none = ~0ull;
+    check(hipMemcpyAsync(L.err.ptr, &none, sizeof(none), hipMemcpyHostToDevice, g_stream), "error word");
+    uint32_t launches = 0;
     const float ms = timed([&] {
-        check(backend == PE_BACKEND_GPU_FP64 ? launch_bs_f64(a, p.risk, grid, g_stream)
-                                             : launch_bs_f32(a, p.risk, grid, g_stream),
-              "bs kernel launch");
+        for (int g = 0; g < 4; ++g) {
+            BsGroup& grp = L.groups[g];
+            if (grp.items.empty()) continue;
+            BsBatchArgs a{};
+            a.trades = L.d_trades.as<BsTrade>();
+            a.obs = L.d_obs.as<BsObs>();
+            a.sobol = L.d_sobol.as<uint32_t>();
+            a.items = grp.d_items.as<BsItem>();
+            a.partials = static_cast<BlockStat*>(L.partials.ptr) + grp.partial_base;
+            a.error = static_cast<unsigned long long*>(L.err.ptr);
+            a.n_items = static_cast<uint32_t>(grp.items.size());
+            a.components = grp.components;
+            check(backend == PE_BACKEND_GPU_FP64 ? launch_bs_batch_f64(a, grp.risk, grp.cont, g_stream)
+                                                 : launch_bs_batch_f32(a, grp.risk, grp.cont, g_stream),
+                  "bs kernel launch");
+            ++launches;
+        }
     });
-    std::vector<BlockStat> host(static_cast<size_t>(grid.x) * grid.y * comps);
-    check(hipMemcpy(host.data(), partials.ptr, host.size() * sizeof(BlockStat), hipMemcpyDeviceToHost), "partials copy");
-    std::memset(out, 0, sizeof(*out));
-    assemble(p, host, grid.x, grid.y, comps, 0, comps, out);
-    out->device_ms = ms;
+    check(hipMemcpy(L.host_partials.data(), L.partials.ptr, L.partial_count * sizeof(BlockStat),
+                    hipMemcpyDeviceToHost),
+          "partials copy");
+    unsigned long long err = none;
+    check(hipMemcpy(&err, L.err.ptr, sizeof(err), hipMemcpyDeviceToHost), "error copy");
+    if (err != none) {
+        const uint64_t trade = err >> 40, unit = err & ((1ull << 40) - 1);
+        throw PlanError(PE_ERR_NONFINITE_STATE, "trade " + std::to_string(trade) + ", sampling unit " +
+                                                    std::to_string(unit) +
+                                                    ": invalid barrier bridge state (non-positive transformed barrier or state)");
+    }
+    for (size_t ti = 0; ti < plans.size(); ++ti) {
+        const BsPlanSlot& slot = L.slots[ti];
+        const BsGroup& grp = L.groups[slot.group];
+        const pe_engine& e = plans[ti]->req.engine;
+        const uint32_t scrambles = e.kind == PE_ENGINE_RQMC ? e.scrambles : 1u;
+        const size_t first = grp.partial_base + static_cast<size_t>(slot.first_partial) * grp.components;
+        std::vector<BlockStat> mine(L.host_partials.begin() + first,
+                                    L.host_partials.begin() + first +
+                                        static_cast<size_t>(scrambles) * slot.chunks * grp.components);
+        pe_result* out = outs + ti;
+        std::memset(out, 0, sizeof(*out));
+        assemble(*plans[ti], mine, slot.chunks, scrambles, grp.components, 0, grp.components, out);
+        out->device_ms = ms;
+    }
     g_kernel_ms = ms;
-    g_kernel_launches = 1;
+    g_kernel_launches = launches;
 }
 
 void upload_lv_tables(HostPlan& p) {
@@ -532,7 +631,8 @@ pe_status pe_plan_evaluate(const pe_plan* plan, pe_backend backend, pe_result* o
             std::vector<HostPlan*> plans{&p};
             evaluate_lv(plans, backend, out);
         } else {
-            evaluate_bs(p, backend, out);
+            std::vector<HostPlan*> plans{&p};
+            evaluate_bs_batch(plans, backend, out);
         }
     });
 }
@@ -548,6 +648,20 @@ pe_status pe_plans_evaluate_shared(const pe_plan* const* plans, uint32_t count, 
             hp.push_back(&const_cast<pe_plan*>(plans[i])->host);
         }
         evaluate_lv(hp, backend, out);
+    });
+}
+
+pe_status pe_batch_evaluate(const pe_plan* const* plans, uint32_t count, pe_backend backend, pe_result* results) {
+    if (!plans || !results || count == 0) return fail(PE_ERR_INVALID_ARGUMENT, "null argument");
+    if (backend != PE_BACKEND_GPU_FP64 && backend != PE_BACKEND_GPU_MIXED) return fail(PE_ERR_INVALID_ARGUMENT, "backend");
+    return guarded([&] {
+        ensure_init();
+        std::vector<HostPlan*> hp;
+        for (uint32_t i = 0; i < count; ++i) {
+            if (!plans[i]) throw PlanError(PE_ERR_INVALID_ARGUMENT, "null plan");
+            hp.push_back(&const_cast<pe_plan*>(plans[i])->host);
+        }
+        evaluate_bs_batch(hp, backend, results);
     });
 }
 

@@ -111,6 +111,83 @@ double HostPlan::forward_at(double t) const {
     return f;
 }
 
+// ------------------------------------------------------------ dividends --
+// AffineDividendTransform::new_with_carry_curves + compile_reserve
+// (crates/pricing/src/market/dividend.rs:318-361, 622-668).
+void HostDividends::compile(double s0, const pe_dividend_quote* q, uint32_t n, const HostCurve* p,
+                            const HostCurve* d) {
+    spot = s0;
+    discount = p;
+    dividend = d;
+    events.clear();
+    for (uint32_t i = 0; i < n; ++i) {
+        const double t = q[i].ex_time, cash = q[i].fixed_cash, beta = q[i].beta;
+        if (!std::isfinite(t) || t < 0.0) invalid("dividend ex-time must be finite and non-negative");
+        if (!(std::isfinite(cash) && cash >= 0.0)) invalid("dividend cash must be finite and non-negative");
+        if (!(std::isfinite(beta) && beta >= 0.0 && beta < 1.0)) invalid("dividend proportion must be in [0, 1)");
+        if (i > 0 && !(t > q[i - 1].ex_time)) invalid("dividend ex-times must increase strictly");
+        const double alpha = cash / s0;
+        if (!std::isfinite(alpha)) invalid("non-finite dividend alpha");
+        events.push_back({t, cash, alpha, beta});
+    }
+    double proportional = 1.0, r = 0.0;
+    for (const Event& e : events) {
+        proportional *= 1.0 - e.beta;
+        const double ratio = std::exp(p->log_discount(e.ex_time)) / std::exp(d->log_discount(e.ex_time));
+        r += e.cash / proportional * ratio;
+    }
+    const double residual = spot - r;
+    if (!std::isfinite(r) || !std::isfinite(residual) || residual <= 0.0) invalid("non-positive escrowed spot");
+    reserve = r;
+    // event_timeline (dividend.rs:524-547)
+    timeline.clear();
+    HostCoord c = initial();
+    double prev = 0.0;
+    long last_cash = -1;
+    for (size_t i = 0; i < events.size(); ++i)
+        if (events[i].cash != 0.0) last_cash = static_cast<long>(i);
+    for (size_t i = 0; i < events.size(); ++i) {
+        const HostCoord before = carry(c, prev, events[i].ex_time);
+        HostCoord after{(1.0 - events[i].beta) * before.a - events[i].alpha, (1.0 - events[i].beta) * before.b,
+                        (1.0 - events[i].beta) * before.ss};
+        if (!std::isfinite(after.a) || !std::isfinite(after.b)) invalid("non-finite dividend transform");
+        if (last_cash < 0 || static_cast<long>(i) >= last_cash) after.a = 0.0;
+        timeline.push_back({events[i].ex_time, before, after});
+        c = after;
+        prev = events[i].ex_time;
+    }
+}
+
+HostCoord HostDividends::initial() const { return HostCoord{reserve / spot, (spot - reserve) / spot, 1.0}; }
+
+HostCoord HostDividends::carry(HostCoord c, double start, double end) const {
+    if (c.a == 0.0 || start == end) return c;
+    const double g = dividend->log_discount(end) - discount->log_discount(end) - dividend->log_discount(start) +
+                     discount->log_discount(start);
+    c.a *= std::exp(g);
+    if (!std::isfinite(c.a)) invalid("non-finite carried affine coordinate");
+    return c;
+}
+
+// coordinate_after_time (dividend.rs:470-497)
+HostCoord HostDividends::after_time(double t) const {
+    HostCoord c = initial();
+    double prev = 0.0;
+    for (const Event& e : events) {
+        if (!(e.ex_time <= t)) break;
+        c = carry(c, prev, e.ex_time);
+        c = HostCoord{(1.0 - e.beta) * c.a - e.alpha, (1.0 - e.beta) * c.b, (1.0 - e.beta) * c.ss};
+        if (!std::isfinite(c.a) || !std::isfinite(c.b)) invalid("non-finite dividend transform");
+        prev = e.ex_time;
+    }
+    c = carry(c, prev, t);
+    bool all = true;
+    for (const Event& e : events)
+        if (!(e.ex_time <= t || e.cash == 0.0)) all = false;
+    if (all) c.a = 0.0;
+    return c;
+}
+
 // ----------------------------------------------------------- bridge order --
 void compile_bridge_program(const std::vector<double>& t, std::vector<BridgeOp>& program,
                             std::vector<double>& inv_sqrt_dt, uint32_t& depth) {
@@ -229,6 +306,112 @@ std::vector<uint32_t> build_sobol_table(uint64_t seed, uint32_t scrambles, uint3
     return table;
 }
 
+// ------------------------------------------------------------ BS trade --
+// Per-observation affine coordinates, pre-dividend coordinates on ex-dates
+// that the payoff monitors (compile/simulation.rs:115-130, 308-338), the
+// continuous-barrier runtime (:339-414) and the kernel's trade descriptor.
+void compile_bs_trade(HostPlan& p, double monitoring_end) {
+    const pe_product& pr = p.req.product;
+    const size_t n = p.obs_times.size();
+    p.obs_coord.assign(n, HostCoord{});
+    p.obs_pre.assign(n, HostCoord{});
+    p.obs_flags.assign(n, 0u);
+    bool has_div = false;
+    for (size_t i = 0; i < n; ++i) {
+        const double t = p.obs_times[i];
+        if (!p.dividends.empty()) p.obs_coord[i] = p.dividends.after_time(t);
+        const HostCoord& c = p.obs_coord[i];
+        const double spot_contract = c.a * p.spot + c.b * p.obs_forwards[i];
+        if (!std::isfinite(spot_contract) || spot_contract <= 0.0) invalid("non-finite dividend forward");
+        if (c.a != 0.0 || c.b != 1.0 || c.ss != 1.0) has_div = true;
+        const HostDividends::Entry* entry = nullptr;
+        for (const auto& e : p.dividends.timeline)
+            if (e.ex_time == t) entry = &e;
+        bool jump = false;
+        if (entry && pr.kind == PE_PRODUCT_BARRIER && !p.continuous && p.obs_days[i] != kNoDay)
+            for (uint32_t k = 0; k < pr.date_count; ++k)
+                if (pr.dates[k] == p.obs_days[i]) jump = true;
+        if (entry && (p.continuous || jump)) {
+            p.obs_flags[i] |= kObsPre;
+            p.obs_pre[i] = entry->before;
+            has_div = true;
+        }
+        if (jump) p.obs_flags[i] |= kObsJump;
+        if (p.obs_weight.size() == n && p.obs_weight[i] != 0.0) p.obs_flags[i] |= kObsFixing;
+        if (p.continuous && t <= monitoring_end) p.obs_flags[i] |= kObsBridge;
+    }
+    p.single_terminal = n == 1 && p.obs_days[0] == pr.expiry_day && !(p.obs_flags[0] & kObsPre);
+    p.resolved = false;
+    p.resolved_survival = 1.0;
+    if (p.continuous) {
+        if (pr.historical_hit == 1) { p.resolved = true; p.resolved_survival = 0.0; }
+        else if (monitoring_end < 0.0) { p.resolved = true; p.resolved_survival = 1.0; }
+        if (p.product.smoothing && !pr.direction_down && pr.historical_hit != 1 && p.dates.back() >= 0) {
+            std::vector<HostCoord> coords{HostCoord{}};
+            coords.insert(coords.end(), p.obs_coord.begin(), p.obs_coord.end());
+            for (size_t i = 0; i < n; ++i)
+                if (p.obs_flags[i] & kObsPre) coords.push_back(p.obs_pre[i]);
+            for (const HostCoord& c : coords)
+                if (p.product.half_width >= pr.barrier - c.a * p.spot)
+                    invalid("smoothing half width reaches the transformed spot barrier");
+        }
+    }
+    p.bs_obs.assign(n, BsObs{});
+    double prev = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        BsObs& o = p.bs_obs[i];
+        o.time = p.obs_times[i];
+        o.sqrt_step = std::sqrt(std::max(o.time - prev, 0.0));
+        prev = o.time;
+        o.forward = p.obs_forwards[i];
+        o.weight = p.obs_weight.size() == n ? p.obs_weight[i] : 0.0;
+        o.a = p.obs_coord[i].a; o.b = p.obs_coord[i].b; o.ss = p.obs_coord[i].ss;
+        o.pa = p.obs_pre[i].a; o.pb = p.obs_pre[i].b; o.pss = p.obs_pre[i].ss;
+        o.flags = p.obs_flags[i];
+    }
+    BsTrade& t = p.trade;
+    std::memset(&t, 0, sizeof(t));
+    t.product = p.product;
+    const pe_engine& e = p.req.engine;
+    t.seed = e.seed;
+    t.units = e.units;
+    t.s0 = p.spot;
+    t.reserve = p.dividends.reserve;
+    const HostCoord c0 = p.dividends.empty() ? HostCoord{} : p.dividends.initial();
+    t.init_a = c0.a;
+    t.init_b = c0.b;
+    t.gamma_bump = p.gamma_bump;
+    t.spot_bump = p.spot_bump;
+    t.vol_bump = p.vol_bump;
+    t.resolved_survival = p.resolved_survival;
+    t.n_obs = static_cast<uint32_t>(n);
+    t.sobol_dims = p.sobol_dims;
+    t.rqmc = e.kind == PE_ENGINE_RQMC ? 1u : 0u;
+    t.antithetic = e.antithetic ? 1u : 0u;
+    t.single_terminal = p.single_terminal ? 1u : 0u;
+    t.request_gamma = p.request_gamma ? 1u : 0u;
+    t.zero_variance = p.total_variance == 0.0 ? 1u : 0u;
+    t.has_dividends = has_div || !p.dividends.empty() ? 1u : 0u;
+    t.continuous = p.continuous ? 1u : 0u;
+    t.resolved = p.resolved ? 1u : 0u;
+    t.expiry_obs = static_cast<uint32_t>(p.expiry_obs);
+    t.risk = p.risk ? 1u : 0u;
+    auto variant = [&](int k, double spot, double vol) { t.variants[k] = Variant{spot, vol, spot / p.spot}; };
+    variant(0, p.spot, p.vol);
+    uint32_t nv = 1;
+    if (p.risk) {
+        variant(1, p.spot - p.spot_bump, p.vol);
+        variant(2, p.spot + p.spot_bump, p.vol);
+        nv = 3;
+        if (p.vol_bump != 0.0) {
+            variant(3, p.spot, p.vol - p.vol_bump);
+            variant(4, p.spot, p.vol + p.vol_bump);
+            nv = 5;
+        }
+    }
+    t.n_variants = nv;
+}
+
 // ------------------------------------------------------------------ plan --
 void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* policy) {
     static std::atomic<uint64_t> next_id{1};
@@ -249,28 +432,59 @@ void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* p
 
     const pe_market& m = r.market;
     if (!std::isfinite(m.spot) || m.spot <= 0.0) invalid("spot must be positive");
-    if (m.dividend_count) unsupported("discrete dividends");
     p.spot = m.spot;
     p.discount_curve.init(m.discount_curve);
     p.dividend_curve.init(m.dividend_curve);
+    if (m.dividend_count) {
+        if (r.model.kind == PE_MODEL_LOCAL_VOL) unsupported("discrete dividends under local volatility");
+        if (!m.dividends) invalid("dividend array missing");
+        p.dividends.compile(p.spot, m.dividends, m.dividend_count, &p.discount_curve, &p.dividend_curve);
+    }
 
-    if (pr.kind == PE_PRODUCT_BARRIER && pr.continuous) unsupported("continuous barrier monitoring");
+    p.continuous = pr.kind == PE_PRODUCT_BARRIER && pr.continuous;
+    if (p.continuous && r.model.kind == PE_MODEL_LOCAL_VOL) unsupported("continuous barriers under local volatility");
     if (!(pr.strike > 0.0) || !(pr.notional > 0.0)) invalid("strike and notional must be positive");
     for (size_t i = 1; i < p.dates.size(); ++i)
         if (!(p.dates[i] > p.dates[i - 1])) invalid("product dates must increase strictly");
-    p.obs_days = fixing_days(pr);
-    if (p.obs_days.empty()) unsupported("fully fixed payoffs");
+    const std::vector<int32_t> contractual = fixing_days(pr);
+    if (contractual.empty()) unsupported("fully fixed payoffs");
+    // Observation points (compile/simulation.rs:210-265): the contractual
+    // dates and, for a continuous barrier, the dividend ex-times up to the
+    // end of monitoring; equal times keep the contractual date.
+    std::vector<std::pair<double, int32_t>> points;
+    for (int32_t d : contractual) points.push_back({static_cast<double>(d) / 365.0, d});
+    double monitoring_end = 0.0;
+    if (p.continuous) {
+        if (p.dates.empty()) invalid("barrier needs monitoring dates");
+        monitoring_end = static_cast<double>(p.dates.back()) / 365.0;
+        std::vector<std::pair<double, int32_t>> kept;
+        for (const auto& pt : points)
+            if (pt.first > 0.0 || pt.second == pr.expiry_day) kept.push_back(pt);
+        points.swap(kept);
+        for (const auto& e : p.dividends.timeline)
+            if (e.ex_time > 0.0 && e.ex_time <= monitoring_end) points.push_back({e.ex_time, kNoDay});
+        std::stable_sort(points.begin(), points.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+        std::vector<std::pair<double, int32_t>> unique;
+        for (const auto& pt : points)
+            if (unique.empty() || unique.back().first != pt.first) unique.push_back(pt);
+        points.swap(unique);
+    }
+    p.obs_days.clear();
     p.obs_times.clear();
-    for (int32_t d : p.obs_days) p.obs_times.push_back(static_cast<double>(d) / 365.0);
+    for (const auto& pt : points) {
+        p.obs_times.push_back(pt.first);
+        p.obs_days.push_back(pt.second);
+    }
     p.expiry = static_cast<double>(pr.expiry_day) / 365.0;
     p.forward = p.forward_at(p.expiry);
     p.discount = std::exp(p.discount_curve.log_discount(static_cast<double>(pr.payment_day) / 365.0));
 
     auto obs_index = [&](int32_t day) -> int32_t {
-        const auto it = std::lower_bound(p.obs_days.begin(), p.obs_days.end(), day);
-        if (it == p.obs_days.end() || *it != day) invalid("missing observation");
-        return static_cast<int32_t>(it - p.obs_days.begin());
+        for (size_t i = 0; i < p.obs_days.size(); ++i)
+            if (p.obs_days[i] == day) return static_cast<int32_t>(i);
+        invalid("missing observation");
     };
+    p.expiry_obs = obs_index(pr.expiry_day);
 
     // Product descriptor and per-observation fixing weights.
     ProductDesc& d = p.product;
@@ -325,11 +539,14 @@ void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* p
             d.down = pr.direction_down ? 1 : 0;
             d.knock_in = pr.knock_in ? 1 : 0;
             d.has_rebate = pr.has_rebate ? 1 : 0;
-            d.rebate = pr.rebate;
+            d.rebate = pr.has_rebate ? pr.rebate : 0.0;
             d.initial_hit = pr.historical_hit == 1 ? 1 : 0;
+            d.continuous = p.continuous ? 1 : 0;
+            if (!(pr.barrier > 0.0) || !std::isfinite(pr.barrier)) invalid("barrier must be positive");
             d.terminal_obs = obs_index(pr.expiry_day);
-            for (uint32_t i = 0; i < pr.date_count; ++i)
-                if (pr.dates[i] >= 0) p.obs_weight[obs_index(pr.dates[i])] = 1.0;
+            if (!p.continuous)
+                for (uint32_t i = 0; i < pr.date_count; ++i)
+                    if (pr.dates[i] >= 0) p.obs_weight[obs_index(pr.dates[i])] = 1.0;
             break;
         case PE_PRODUCT_LOOKBACK:
             d.kind = kLookback;
@@ -438,14 +655,18 @@ void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* p
     p.request_gamma = r.risk.gamma != 0;
     p.risk = r.risk.delta || r.risk.vega || p.request_gamma;
     if (p.lv && p.risk) unsupported("local-vol pathwise risk");
-    if (p.risk && !p.lv && (pr.kind == PE_PRODUCT_DIGITAL || pr.kind == PE_PRODUCT_BARRIER) && !d.smoothing)
+    if (p.risk && !p.lv && pr.kind == PE_PRODUCT_DIGITAL && !d.smoothing)
         invalid("pathwise risk of a discontinuous payoff needs compact-C2 smoothing");
+    if (p.risk && !p.lv && pr.kind == PE_PRODUCT_BARRIER && !p.continuous && !d.smoothing)
+        invalid("pathwise risk of a discontinuous payoff needs compact-C2 smoothing");
+    const double residual_spot = p.spot - p.dividends.reserve;
     if (p.request_gamma) p.gamma_bump = r.risk.gamma_relative ? p.spot * r.risk.gamma_bump : r.risk.gamma_bump;
-    if (p.request_gamma && (!std::isfinite(p.gamma_bump) || !(p.gamma_bump > 0.0) || p.gamma_bump >= p.spot))
-        invalid("Gamma bump must be positive and below the spot");
-    p.spot_bump = p.request_gamma ? p.gamma_bump : std::min(p.spot * kRelativeSpotBump, 0.5 * p.spot);
+    if (p.request_gamma && (!std::isfinite(p.gamma_bump) || !(p.gamma_bump > 0.0) || p.gamma_bump >= residual_spot))
+        invalid("Gamma bump must be positive and below the escrowed spot");
+    p.spot_bump = p.request_gamma ? p.gamma_bump : std::min(p.spot * kRelativeSpotBump, 0.5 * residual_spot);
     p.vol_bump = p.vol == 0.0 ? 0.0 : std::min(kVolatilityBump, p.vol * 0.5);
     p.sobol_dims = p.lv ? static_cast<uint32_t>(p.nodes.size() - 1) : static_cast<uint32_t>(p.obs_times.size());
+    if (!p.lv) compile_bs_trade(p, monitoring_end);
 }
 
 }  // namespace pe

@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -78,17 +79,56 @@ struct LvLaunch {
     }
 };
 
+// Escrowed affine discrete dividends (crates/pricing/src/market/dividend.rs:96-547):
+// S = A*S0 + B*F, cash funded by a reserve carried on the market curves.
+struct HostCoord {
+    double a = 0.0, b = 1.0, ss = 1.0;
+};
+
+struct HostDividends {
+    struct Event {
+        double ex_time, cash, alpha, beta;
+    };
+    struct Entry {
+        double ex_time;
+        HostCoord before, after;
+    };
+    double spot = 0.0, reserve = 0.0;
+    std::vector<Event> events;
+    std::vector<Entry> timeline;
+    const HostCurve* discount = nullptr;
+    const HostCurve* dividend = nullptr;
+
+    bool empty() const { return events.empty(); }
+    void compile(double s0, const pe_dividend_quote* q, uint32_t n, const HostCurve* p, const HostCurve* d);
+    HostCoord initial() const;
+    HostCoord carry(HostCoord c, double start, double end) const;
+    HostCoord after_time(double t) const;
+};
+
+struct BsLaunch;
+struct BsLaunchDeleter {
+    void operator()(BsLaunch* l) const;
+};
+
 struct HostPlan {
     uint64_t id = 0;                 // unique per compiled plan, never reused
     std::unique_ptr<LvLaunch> lv_launch;
+    std::unique_ptr<BsLaunch, BsLaunchDeleter> bs_launch;
     pe_request req{};
     std::vector<int32_t> dates;
     std::vector<double> weights, fixings;
 
     HostCurve discount_curve, dividend_curve;
+    HostDividends dividends;
     double spot = 0.0;
-    std::vector<int32_t> obs_days;
+    std::vector<int32_t> obs_days;        // kNoDay: dividend-only bridge point
     std::vector<double> obs_times, obs_forwards;
+    std::vector<HostCoord> obs_coord, obs_pre;
+    std::vector<uint32_t> obs_flags;      // BsObsFlags
+    bool continuous = false, resolved = false, single_terminal = false;
+    double resolved_survival = 1.0;
+    int32_t expiry_obs = -1;
     double expiry = 0.0, forward = 0.0, discount = 1.0;
     double vol = 0.0, total_variance = 0.0;
     double gamma_bump = 0.0, spot_bump = 0.0, vol_bump = 0.0;
@@ -112,6 +152,10 @@ struct HostPlan {
     // RQMC
     uint32_t sobol_dims = 0;
 
+    // BS/Black-76 trade descriptor and observation table for the batch kernel
+    BsTrade trade{};
+    std::vector<BsObs> bs_obs;
+
     // device tables
     DeviceBuffer d_obs, d_sobol, d_steps, d_program, d_grid, d_grid32, d_x, d_x32, d_node_weight;
 
@@ -124,9 +168,11 @@ void compile_bridge_program(const std::vector<double>& times, std::vector<Bridge
                             std::vector<double>& inv_sqrt_dt, uint32_t& depth);
 const uint32_t* direction_words(uint32_t* dimensions);  // embedded Joe--Kuo asset (host order)
 
+constexpr int32_t kNoDay = INT32_MIN;
+
 // Launchers (pe_kernels_f64.hip / pe_kernels_f32.hip).
-hipError_t launch_bs_f64(const BsArgs& a, bool risk, dim3 grid, hipStream_t s);
-hipError_t launch_bs_f32(const BsArgs& a, bool risk, dim3 grid, hipStream_t s);
+hipError_t launch_bs_batch_f64(const BsBatchArgs& a, bool risk, bool cont, hipStream_t s);
+hipError_t launch_bs_batch_f32(const BsBatchArgs& a, bool risk, bool cont, hipStream_t s);
 hipError_t launch_lv_f64(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s);
 hipError_t launch_lv_f32(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s);
 hipError_t launch_dupire(const double* theta_t, const double* theta_v, const double* theta_d, uint32_t nk,

@@ -8,9 +8,10 @@
 namespace pe {
 
 constexpr int kMaxProducts = 4;      // products sharing one simulated path set
-constexpr int kMaxVariants = 7;      // base, Gamma +/-, validation spot +/-, vol +/-
+constexpr int kMaxVariants = 5;      // base, validation (= Gamma) spot +/-, vol +/-
 constexpr int kMaxBridgeDepth = 64;  // streaming Brownian-bridge stack (LDS)
 constexpr int kBlockThreads = 256;
+constexpr int kComponents = 18;      // the 18-wide pathwise risk vector (PE_COMPONENTS)
 
 enum ProductKind : int32_t { kEuropean = 0, kDigital = 1, kBarrier = 2, kAsian = 3, kLookback = 4 };
 
@@ -28,7 +29,7 @@ struct ProductDesc {
     int32_t has_hist_ext;    // lookback historical extremum
     int32_t terminal_obs;    // observation index of the expiry fixing
     int32_t first_obs;       // lookback: first future observation
-    int32_t pad;
+    int32_t continuous;      // barrier monitored continuously (bridge survival)
     double strike, notional, payout, barrier, rebate, half_width;
     double discount;         // DF(payment)
     double hist_ext;         // lookback historical extremum
@@ -36,14 +37,22 @@ struct ProductDesc {
     float f_barrier, f_half_width, f_strike, f_pad;  // FP32 copies for the mixed kernels
 };
 
-// One observation (BS kernel) or one grid node (LV kernel).  mask bit p: the
-// node is a fixing of product p; weight[p] is its Asian weight.
-struct ObsDesc {
-    double time;         // year fraction
-    double sqrt_step;    // sqrt(max(t_i - t_{i-1}, 0)) (BS path)
-    double forward;      // F(t_i)
-    double weight[kMaxProducts];
-    uint32_t mask;
+// One observation of a BS/Black-76 trade: time, Brownian step, forward and
+// the escrowed-dividend affine coordinates S = A*S0 + B*F after (and, on an
+// ex-date, before) the dividend (crates/pricing/src/market/dividend.rs:136-205,
+// crates/pricing/src/engine/processes/black_scholes.rs:335-354).
+enum BsObsFlags : uint32_t {
+    kObsPre = 1u,       // pre-dividend coordinate present (ex-date)
+    kObsFixing = 2u,    // fixing of the product (Asian weight, barrier or lookback date)
+    kObsJump = 4u,      // discrete barrier monitors the pre-dividend spot too
+    kObsBridge = 8u,    // continuous barrier: bridge observation
+};
+
+struct BsObs {
+    double time, sqrt_step, forward, weight;
+    double a, b, ss;          // post-dividend coordinate (A, B, spot scale)
+    double pa, pb, pss;       // pre-dividend coordinate
+    uint32_t flags;
     uint32_t pad;
 };
 
@@ -128,28 +137,50 @@ struct ProductHot {
     float half_width;
 };
 
-// Kernel arguments of the BS/Black-76 pathwise kernel.
-struct BsArgs {
-    const ObsDesc* obs;
-    const uint32_t* sobol;     // [scramble][dim][33]: 32 pre-scrambled directions + shift
-    void* partials;            // BlockStat [scramble][block][component]
+// One BS/Black-76 trade of a batch (one compiled plan).
+struct BsTrade {
     ProductDesc product;
     Variant variants[kMaxVariants];
     uint64_t seed;
     uint64_t units;            // pseudo units or RQMC points per scramble
     double s0;
-    double expiry;             // T (single-terminal branch)
+    double reserve;            // escrowed cash reserve (0 without cash dividends)
+    double init_a, init_b;     // initial affine coordinate (continuous barrier)
     double gamma_bump, spot_bump, vol_bump;
-    uint32_t n_obs;
-    uint32_t n_variants;
-    uint32_t rqmc;
-    uint32_t antithetic;
-    uint32_t risk;
-    uint32_t single_terminal;
-    uint32_t request_gamma;
-    uint32_t zero_variance;
-    uint32_t components;       // statistics written per unit
+    double resolved_survival;  // continuous barrier decided by history
+    uint32_t obs_offset, n_obs;
+    uint32_t sobol_offset;     // words into the batch Sobol table
     uint32_t sobol_dims;
+    uint32_t n_variants;
+    uint32_t rqmc, antithetic, single_terminal;
+    uint32_t request_gamma, zero_variance;
+    uint32_t has_dividends;    // any non-identity coordinate
+    uint32_t continuous;       // continuous barrier monitoring
+    uint32_t resolved;         // continuous barrier: survival fixed by history
+    uint32_t expiry_obs;
+    uint32_t risk;
+    uint32_t pad;
+};
+
+// A unit range of one trade and scramble, reduced by one workgroup into one
+// BlockStat per component at `partial`.
+struct BsItem {
+    uint32_t trade;
+    uint32_t scramble;
+    uint64_t unit_begin, unit_end;
+    uint32_t partial;
+    uint32_t pad;
+};
+
+struct BsBatchArgs {
+    const BsTrade* trades;
+    const BsObs* obs;
+    const uint32_t* sobol;
+    const BsItem* items;
+    void* partials;            // BlockStat [partial][component]
+    unsigned long long* error; // smallest (trade << 40 | unit) whose path failed
+    uint32_t n_items;
+    uint32_t components;       // statistics per item (1, 10 or 18)
 };
 
 // Kernel arguments of the local-vol Log-Euler kernel.

@@ -197,6 +197,7 @@ PE_SYMBOLS = {
     "pe_plan_free": (None, [C.c_void_p]),
     "pe_plan_evaluate": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(pe_result)]),
     "pe_plans_evaluate_shared": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.c_int, C.POINTER(pe_result)]),
+    "pe_batch_evaluate": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.c_int, C.POINTER(pe_result)]),
     "pe_local_variance_from_ssvi": (
         C.c_int,
         [C.POINTER(pe_ssvi_surface), C.c_uint32, _DP, C.c_uint32, _DP, C.c_double, C.c_double, _DP,

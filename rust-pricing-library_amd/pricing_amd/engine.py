@@ -93,6 +93,15 @@ def evaluate_shared(plans, backend="fp64"):
     return list(results)
 
 
+def evaluate_batch(plans, backend="fp64"):
+    """Evaluates independent BS/Black-76 plans (a trade batch) in persistent launches."""
+    n = len(plans)
+    handles = (C.c_void_p * n)(*[p._handle.value for p in plans])
+    results = (abi.pe_result * n)()
+    _check(load().pe_batch_evaluate(handles, n, BACKENDS[backend], results))
+    return list(results)
+
+
 def local_variance_from_ssvi(theta_times, theta_values, terminal_slope, rho, eta, gamma, time_nodes, x_nodes,
                              floor, cap):
     """Dupire local variance of a StandardSsvi surface, built in FP64 on the GPU."""

@@ -88,3 +88,90 @@ def lv_path_dependent(values_grid, points=1 << 16, scrambles=16, seed=0xFEDCBA98
         req.set_risk()
         reqs.append(req)
     return reqs
+
+
+# ------------------------------------------------------------ trade batch --
+class SplitMix64:
+    """SplitMix64 (Steele, Lea, Flood 2014): the batch generator's stream."""
+
+    def __init__(self, seed):
+        self.state = seed & 0xFFFFFFFFFFFFFFFF
+
+    def next(self):
+        self.state = (self.state + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        return z ^ (z >> 31)
+
+    def uniform(self, lo=0.0, hi=1.0):
+        return lo + (hi - lo) * ((self.next() >> 11) * (1.0 / (1 << 53)))
+
+    def integer(self, lo, hi):
+        """Uniform integer in [lo, hi]."""
+        return lo + self.next() % (hi - lo + 1)
+
+
+BATCH_RATE = 0.03
+BATCH_HALF_WIDTH = 1.0
+
+
+def batch_trades(count=4096, seed=0x00C0FFEE, min_log2=12.0, max_log2=18.0, risk=True):
+    """Batched uneven BS/Black-76 trades with affine discrete dividends: trade
+    id mod 5 is European / cash Digital / Barrier (discrete for odd ids,
+    continuous for even ids) / arithmetic Asian / fixed Lookback; call and put
+    alternate; K ~ U[70, 130], sigma ~ U[0.1, 0.5], T ~ U[0.1, 5] years
+    (expiry = round(365 T) days); 12-365 equally spaced dates (Europeans and
+    Digitals: one); pseudo-MC paths round(2^U[min_log2, max_log2]) with a seed
+    per trade; 0-4 dividends of 1.0 cash + 1% proportional at U(0, T) ex-days;
+    up-and-out call barrier max(S0, K) U[1.1, 1.4] or down-and-out put barrier
+    min(S0, K) U[0.6, 0.9]; compact-C2 smoothing of half width 1.0; r = 3%,
+    q = 0; BS for even ids, Black-76 for odd ids; price with Delta and Vega."""
+    rng = SplitMix64(seed)
+    s0 = 100.0
+    out = []
+    for tid in range(count):
+        kind = tid % 5
+        call = tid % 2 == 0
+        strike = rng.uniform(70.0, 130.0)
+        vol = rng.uniform(0.1, 0.5)
+        t_years = rng.uniform(0.1, 5.0)
+        expiry = max(1, int(round(365.0 * t_years)))
+        steps = rng.integer(12, 365)
+        paths = int(round(2.0 ** rng.uniform(min_log2, max_log2)))
+        n_div = rng.integer(0, 4)
+        ex_days = sorted({rng.integer(1, max(1, expiry - 1)) for _ in range(n_div)}) if expiry > 1 else []
+        barrier_u = rng.uniform(0.0, 1.0)
+        seed_t = rng.next()
+        horizon = expiry / 365.0
+        req = request.Request()
+        req.set_market(s0, [0.0, horizon], [1.0, math.exp(-BATCH_RATE * horizon)], [0.0, horizon], [1.0, 1.0],
+                       [(d / 365.0, 1.0, 0.01) for d in ex_days])
+        req.set_black_scholes(vol, black76=tid % 2 == 1)
+        side = abi.PE_CALL if call else abi.PE_PUT
+        n = min(steps, expiry)
+        dates = sorted({max(1, int(round(expiry * (k + 1) / n))) for k in range(n)})
+        if kind == 0:
+            req.set_product(abi.PE_PRODUCT_EUROPEAN, side, expiry, strike)
+        elif kind == 1:
+            req.set_product(abi.PE_PRODUCT_DIGITAL, side, expiry, strike, payout=1.0)
+        elif kind == 2:
+            if call:
+                barrier = max(s0, strike) * (1.1 + 0.3 * barrier_u)
+                down = 0
+            else:
+                barrier = min(s0, strike) * (0.6 + 0.3 * barrier_u)
+                down = 1
+            req.set_product(abi.PE_PRODUCT_BARRIER, side, expiry, strike, barrier=barrier, direction_down=down,
+                            continuous=int(tid % 2 == 0), dates=dates)
+        elif kind == 3:
+            w = [1.0 / len(dates)] * len(dates)
+            req.set_product(abi.PE_PRODUCT_ASIAN, side, expiry, strike, dates=dates, weights=w,
+                            fixings=[None] * len(dates))
+        else:
+            req.set_product(abi.PE_PRODUCT_LOOKBACK, side, expiry, strike, dates=dates)
+        req.set_pseudo(seed_t, paths)
+        req.set_risk(delta=risk, vega=risk, smoothing_half_width=BATCH_HALF_WIDTH)
+        req.trade_id = tid
+        out.append(req)
+    return out

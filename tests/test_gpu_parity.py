@@ -72,7 +72,7 @@ def test_european_bs_mixed_within_tenth_se():
 
 
 GPU_REPLAY = [c for c in golden()["replay"] if c["name"] not in (
-    "pseudo_mc_delta_gamma_vega_vegakt", "rqmc_delta_gamma_vega_vegakt", "continuous_barrier_smoothed_full_risk")]
+    "pseudo_mc_delta_gamma_vega_vegakt", "rqmc_delta_gamma_vega_vegakt")]
 
 
 @pytest.mark.parametrize("case", GPU_REPLAY, ids=[c["name"] for c in GPU_REPLAY])
