@@ -46,6 +46,8 @@ def compare(req, gpu, ref, comps, rel=REL):
         g, r = gpu.estimates[c], ref.estimates[c]
         scale = _scale(req, ref, c)
         assert abs(g.value - r.value) <= rel * scale + 1e-15, (abi.COMPONENT_NAMES[c], g.value, r.value)
+        if c in DIAG:
+            continue  # the device reports path diagnostics as means only
         se_scale = scale / max(abs(r.value), 1e-300) * r.standard_error
         assert abs(g.standard_error - r.standard_error) <= 1e-10 * max(se_scale, r.standard_error) + 1e-15, (
             abi.COMPONENT_NAMES[c], g.standard_error, r.standard_error)
