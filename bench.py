@@ -11,6 +11,10 @@ Workload (one "step" = one evaluation of the whole job):
   products: up-and-out call K=100, B=130 with daily monitoring and an
   arithmetic Asian call K=100, priced on the same paths in one launch.
 
+A second block ("batch_bs") times the batched BS/Black-76 workload: 4096
+uneven trades with affine dividends and Delta/Vega, mixed vs FP64 device modes
+and the CPU oracle on a sample of the trades.
+
 Prints one JSON line.  --gpus above 1 is refused: this build runs one GPU.
 """
 import argparse
@@ -107,6 +111,73 @@ def cpu_baseline(reqs, target_seconds):
                       f"barrier + Asian on shared paths, FP64 oracle, {dt:.2f} s wall"}
 
 
+def batch_block(steps, cpu_seconds, no_cpu, no_fp64):
+    """Config 3: 4096 uneven BS/Black-76 trades with affine dividends, Delta and
+    Vega, one launch per kernel flavour; mixed vs FP64 device modes and the CPU
+    oracle on a sample of the trades."""
+    reqs = workloads.batch_trades(4096)
+    plans = [engine.PricingPlan.compile(r) for r in reqs]
+    engine.evaluate_batch(plans, "mixed")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        mixed = engine.evaluate_batch(plans, "mixed")
+    wall = (time.perf_counter() - t0) / steps
+    kernel_ms = engine.last_kernel_ms()[0]
+    path_steps = sum(r.path_steps for r in mixed)
+    block = {
+        "workload": "4096 trades (European / cash Digital / discrete or continuous Barrier / Asian / Lookback), "
+                    "K~U[70,130], sigma~U[0.1,0.5], T~U[0.1,5]y, 12-365 dates, 2^U[12,18] pseudo paths, "
+                    "0-4 dividends (1.0 cash + 1%), compact-C2 half width 1.0, price + Delta + Vega",
+        "path_steps": path_steps,
+        "paths": sum(r.evaluated_paths for r in mixed),
+        "value": path_steps / wall,
+        "unit": "path-steps/s",
+        "ms_per_step": 1e3 * wall,
+        "kernel_ms": kernel_ms,
+    }
+    if not no_fp64:
+        engine.evaluate_batch(plans, "fp64")
+        t0 = time.perf_counter()
+        f64 = engine.evaluate_batch(plans, "fp64")
+        wall64 = time.perf_counter() - t0
+        block["fp64_value"] = path_steps / wall64
+        block["fp64_kernel_ms"] = engine.last_kernel_ms()[0]
+        block["mixed_over_fp64"] = wall64 / wall
+        worst = {}
+        for name, c in (("price", 0), ("delta", 1), ("vega", 2)):
+            worst[name] = max(abs(a.estimates[c].value - b.estimates[c].value) / b.estimates[c].standard_error
+                              for a, b in zip(mixed, f64) if b.estimates[c].standard_error > 0)
+        block["max_mixed_minus_fp64_in_se"] = worst
+    if not no_cpu:
+        from concurrent.futures import ThreadPoolExecutor
+
+        from _support import oracle_evaluate, oracle
+
+        cores = int(oracle().oracle_hardware_threads())
+        sample = [r for r in reqs if r.trade_id % 64 == 0]
+        full_units = [r.c.engine.units for r in sample]
+        # one trade per core at a time (the oracle releases the GIL); trades
+        # longest-first so the pool drains evenly
+        order = sorted(range(len(sample)), key=lambda i: -full_units[i] * max(1, sample[i].c.product.date_count))
+        scale = 64
+        with ThreadPoolExecutor(max_workers=cores) as pool:
+            while True:
+                for r, n in zip(sample, full_units):
+                    r.c.engine.units = max(256, n // scale)
+                t0 = time.perf_counter()
+                cpu_steps = sum(pool.map(lambda i: oracle_evaluate(sample[i], 1, 4096).path_steps, order))
+                dt = time.perf_counter() - t0
+                if dt >= 0.25 * cpu_seconds or scale == 1:
+                    break
+                scale = max(1, scale // max(2, min(16, int(cpu_seconds / max(dt, 1e-3)))))
+        block["cpu_baseline"] = {
+            "value": cpu_steps / dt, "unit": "path-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{len(sample)} of the 4096 trades (every 64th) at 1/{scale} of their path counts "
+                      f"({cpu_steps} path-steps), FP64 oracle, one trade per core, {dt:.2f} s wall"}
+        block["speedup_vs_cpu_all_cores"] = block["value"] / block["cpu_baseline"]["value"]
+    return block
+
+
 def copy_probe(nbytes):
     """A plain device-to-device copy of known size: the HBM counters it reads
     (FETCH_SIZE, WRITE_SIZE) calibrate the per-dispatch traffic figures."""
@@ -134,6 +205,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 4096-trade BS/Black-76 batch block")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -213,6 +285,8 @@ def main():
         out["speedup_vs_cpu_all_cores"] = value / cb["value"]
     else:
         out["cpu_baseline"] = None
+    if not args.no_batch:
+        out["batch_bs"] = batch_block(3, args.cpu_seconds, args.no_cpu, args.no_fp64)
     print(json.dumps(out))
     return 0
 
