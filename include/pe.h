@@ -265,6 +265,40 @@ pe_status pe_plans_evaluate_shared(const pe_plan* const* plans, uint32_t count,
 pe_status pe_batch_evaluate(const pe_plan* const* plans, uint32_t count, pe_backend backend,
                             pe_result* results);
 
+/* ------------------------------------------------- lifted Heston (rough) -- */
+
+/* LiftedHeston (crates/pricing/src/models/rough_volatility.rs:172-212): a
+ * finite exponential-kernel Heston, V = v0 + sum_i w_i f_i, every factor
+ * driven by one variance Brownian motion; rates x_i in inverse years. */
+typedef struct pe_lifted_heston {
+    double initial_variance;
+    double mean_reversion;
+    double long_run_variance;
+    double vol_of_vol;
+    double correlation;
+    uint32_t factor_count;          /* 1..256 */
+    const double* weights;
+    const double* rates;
+} pe_lifted_heston;
+
+/* LiftedHeston::from_rough (crates/pricing/src/models/rough_volatility.rs:564-606):
+ * Abi Jaber's geometric-bin weights and rates of the fractional kernel for
+ * Hurst index `hurst`, `factors` factors and bin ratio `ratio` (> 1). */
+pe_status pe_lifted_heston_from_rough(double hurst, uint32_t factors, double ratio, double* weights_out,
+                                      double* rates_out);
+
+/* RoughVolatilityPricingPlan::compile (crates/pricing/src/engine/risk/rough_volatility.rs:84-190)
+ * for a lifted Heston model.  The request is a Black-Scholes carrier of market,
+ * payoff, dates and engine (its volatility is not used); risk must be off and
+ * the market must have no discrete dividends.  The time grid is the
+ * local-volatility grid of the events {0, expiry, fixings} with at most
+ * `maximum_step` between nodes.  Evaluate with pe_plan_evaluate (the
+ * reference's per-plan grid) or pe_plans_evaluate_shared (several plans on one
+ * simulated path set over the union of their events; no reference
+ * counterpart). */
+pe_status pe_rough_plan_compile(const pe_request* carrier, const pe_lifted_heston* model, double maximum_step,
+                                const pe_exec_policy* policy, pe_plan** out);
+
 /* LocalVarianceGrid::from_surface for a StandardSsvi surface with a ThetaPchip
  * total-variance curve and a power-law phi, computed in FP64 on the device
  * (crates/pricing/src/market/local_variance.rs:149-206, ssvi.rs:61-186,226-275,

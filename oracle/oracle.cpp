@@ -17,6 +17,7 @@
 // LV path:          crates/pricing/src/engine/processes/local_vol.rs:459-668
 // LV valuation:     crates/pricing/src/engine/processes/local_vol_valuation.rs:161-247
 // Estimates:        crates/pricing/src/engine/risk/report.rs:454-473
+// Rough (lifted Heston): crates/pricing/src/engine/risk/rough_volatility.rs:84-342
 //
 // Parity status per function is recorded in tests/test_oracle_*.py.
 #include <algorithm>
@@ -36,6 +37,7 @@
 #include "oracle_market.hpp"
 #include "oracle_numerics.hpp"
 #include "oracle_payoff.hpp"
+#include "oracle_rough.hpp"
 #include "oracle_sampling.hpp"
 #include "pe.h"
 
@@ -798,6 +800,188 @@ static void evaluate_plan(const Plan& p, pe_result* out) {
     out->path_steps = out->evaluated_paths * out->time_steps;
 }
 
+// ------------------------------------------------ lifted Heston pricing --
+// RoughVolatilityPricingPlan (crates/pricing/src/engine/risk/rough_volatility.rs:84-342)
+// for a lifted Heston model and deterministic rates, without discrete
+// dividends.  Several carriers may share one path set (the union of their
+// events is the grid); with one carrier this is the reference's plan.
+struct RoughPlan {
+    std::vector<std::unique_ptr<Plan>> bases;
+    LiftedHeston model;
+    TimeGrid grid;
+    std::vector<double> scale;                  // physical spot = scale * F per node
+    std::vector<std::vector<size_t>> obs_node;  // per carrier: grid node of each observation
+    pe_engine engine{};
+    uint64_t block = 4096;
+    unsigned workers = 1;
+};
+
+static void compile_rough(RoughPlan& rp, const pe_request* const* reqs, size_t count, const pe_lifted_heston& m,
+                          double maximum_step, const pe_exec_policy* policy) {
+    if (count == 0) throw std::invalid_argument("no carrier");
+    rp.model.v0 = m.initial_variance;
+    rp.model.kappa = m.mean_reversion;
+    rp.model.theta = m.long_run_variance;
+    rp.model.nu = m.vol_of_vol;
+    rp.model.rho = m.correlation;
+    if (!m.weights || !m.rates) throw std::invalid_argument("lifted Heston arrays missing");
+    rp.model.weights.assign(m.weights, m.weights + m.factor_count);
+    rp.model.rates.assign(m.rates, m.rates + m.factor_count);
+    rp.model.validate();
+    std::vector<double> events;
+    for (size_t k = 0; k < count; ++k) {
+        const pe_request& r = *reqs[k];
+        if (r.model.kind != PE_MODEL_BLACK_SCHOLES)
+            throw std::invalid_argument("rough extension requires a Black-Scholes request carrier");
+        if (r.risk.delta || r.risk.vega || r.risk.gamma) throw std::invalid_argument("rough pricing is price-only");
+        if (r.market.dividend_count) throw std::invalid_argument("discrete dividends under lifted Heston are not supported");
+        if (r.product.kind == PE_PRODUCT_BARRIER && r.product.continuous)
+            throw std::invalid_argument("continuous barriers are not supported by the rough adapter");
+        auto base = std::make_unique<Plan>();
+        compile_plan(*base, r, policy);
+        const double expiry = static_cast<double>(r.product.expiry_day) / 365.0;
+        if (!std::isfinite(expiry) || expiry <= 0.0 || !std::isfinite(maximum_step) || maximum_step <= 0.0)
+            throw std::invalid_argument("rough positive horizon and step");
+        events.insert(events.end(), base->obs_times.begin(), base->obs_times.end());
+        events.push_back(0.0);
+        events.push_back(expiry);
+        if (k == 0) {
+            rp.engine = r.engine;
+            rp.block = base->block;
+            rp.workers = base->workers;
+        } else {
+            const pe_engine& e = r.engine;
+            if (e.kind != rp.engine.kind || e.seed != rp.engine.seed || e.units != rp.engine.units ||
+                e.scrambles != rp.engine.scrambles || e.antithetic != rp.engine.antithetic ||
+                e.brownian_bridge != rp.engine.brownian_bridge || r.market.spot != reqs[0]->market.spot)
+                throw std::invalid_argument("shared rough plans need one engine and spot");
+        }
+        rp.bases.push_back(std::move(base));
+    }
+    std::sort(events.begin(), events.end());
+    events.erase(std::unique(events.begin(), events.end()), events.end());
+    double estimated = 0.0;
+    for (size_t i = 0; i + 1 < events.size(); ++i) estimated += std::ceil((events[i + 1] - events[i]) / maximum_step);
+    if (!std::isfinite(estimated) || estimated > 65536.0) throw std::invalid_argument("rough time grid resource limit");
+    rp.grid.compile(events, maximum_step);
+    const Forward& mk = rp.bases[0]->market;
+    rp.scale.clear();
+    for (double t : rp.grid.nodes) {
+        // HullWhiteDividendPlan::new node scale without dividends
+        // (crates/pricing/src/models/hull_white_dividends.rs:228-235)
+        const double p = mk.discount_curve.discount(t);
+        const double q = mk.dividend_curve.discount(t);
+        const double past_beta = 1.0;
+        rp.scale.push_back(past_beta * mk.spot / mk.spot * q / p);
+    }
+    rp.obs_node.clear();
+    for (const auto& b : rp.bases) {
+        std::vector<size_t> nodes;
+        for (double t : b->obs_times) {
+            auto it = std::lower_bound(rp.grid.nodes.begin(), rp.grid.nodes.end(), t);
+            if (it == rp.grid.nodes.end() || *it != t) throw std::runtime_error("missing observation time");
+            nodes.push_back(static_cast<size_t>(it - rp.grid.nodes.begin()));
+        }
+        rp.obs_node.push_back(nodes);
+    }
+}
+
+// RoughVolatilityPricingPlan::sample for every carrier on one normal vector.
+static void rough_sample(const RoughPlan& rp, std::vector<double>& normals, const BridgePlan* bridge, double* out) {
+    const size_t n = rp.grid.steps();
+    if (bridge) {
+        for (size_t blk = 0; blk < 2; ++blk) {
+            std::vector<double> part(normals.begin() + blk * n, normals.begin() + (blk + 1) * n);
+            const std::vector<double> t = bridge->apply_one_factor(part);
+            std::copy(t.begin(), t.end(), normals.begin() + blk * n);
+        }
+    }
+    const bool anti = rp.engine.antithetic != 0;
+    const double signs[2] = {1.0, -1.0};
+    const int count = anti ? 2 : 1;
+    std::vector<double> z(normals.size()), forwards, variance, factors, post;
+    const size_t nb = rp.bases.size();
+    for (size_t k = 0; k < nb; ++k) out[k] = 0.0;
+    for (int s = 0; s < count; ++s) {
+        for (size_t i = 0; i < z.size(); ++i) z[i] = signs[s] * normals[i];
+        lifted_heston_path(rp.model, rp.grid.nodes, rp.bases[0]->market.spot, z.data(), forwards, variance, factors);
+        for (size_t k = 0; k < nb; ++k) {
+            const Plan& b = *rp.bases[k];
+            post.resize(rp.obs_node[k].size());
+            for (size_t i = 0; i < post.size(); ++i) {
+                const size_t node = rp.obs_node[k][i];
+                post[i] = rp.scale[node] * forwards[node] + 0.0;
+                if (!std::isfinite(post[i])) throw std::runtime_error("rough observation overflow");
+            }
+            b.tape.forward(post.data(), post.data(), tl_values);
+            const double payoff = b.discount * tl_values[b.tape.output];
+            out[k] += payoff / static_cast<double>(count);
+        }
+    }
+}
+
+static void evaluate_rough(const RoughPlan& rp, pe_result* outs) {
+    const size_t nb = rp.bases.size();
+    const size_t n = rp.grid.steps();
+    const uint32_t dim = static_cast<uint32_t>(2 * n);
+    std::unique_ptr<BridgePlan> bridge;
+    if (rp.engine.brownian_bridge) {
+        bridge.reset(new BridgePlan());
+        bridge->compile(rp.grid.nodes);
+    }
+    const pe_engine& e = rp.engine;
+    std::vector<Statistics> stats;
+    uint64_t independent;
+    uint64_t paths;
+    if (e.kind == PE_ENGINE_PSEUDO) {
+        stats = map_reduce_statistics_n(e.units, rp.block, rp.workers, static_cast<int>(nb), [&](uint64_t u, double* v) {
+            std::vector<double> z(dim);
+            for (uint32_t d = 0; d < dim; ++d) z[d] = philox_normal(e.seed, u, d, kValuation);
+            rough_sample(rp, z, bridge.get(), v);
+        });
+        independent = e.units;
+        paths = e.units * (e.antithetic ? 2 : 1);
+    } else {
+        RqmcPlan q;
+        q.compile(g_directions, e.units, e.scrambles, e.seed, dim);
+        std::vector<std::vector<double>> means(nb);
+        for (uint32_t s = 0; s < e.scrambles; ++s) {
+            auto within = map_reduce_statistics_n(e.units, rp.block, rp.workers, static_cast<int>(nb),
+                                                  [&](uint64_t u, double* v) {
+                                                      std::vector<double> z(dim);
+                                                      for (uint32_t d = 0; d < dim; ++d)
+                                                          z[d] = inverse_standard_normal(q.uniform(s, u, d));
+                                                      rough_sample(rp, z, bridge.get(), v);
+                                                  });
+            for (size_t k = 0; k < nb; ++k) means[k].push_back(within[k].sum.total() / static_cast<double>(e.units));
+        }
+        stats.resize(nb);
+        for (size_t k = 0; k < nb; ++k) stats[k] = Statistics::two_pass(means[k]);
+        independent = e.scrambles;
+        paths = e.units * e.scrambles * (e.antithetic ? 2 : 1);
+    }
+    for (size_t k = 0; k < nb; ++k) {
+        pe_result* out = outs + k;
+        std::memset(out, 0, sizeof(*out));
+        if (!stats[k].moments.has_variance()) throw std::invalid_argument("insufficient sampling units");
+        const double var = stats[k].moments.sample_variance();
+        const double value = stats[k].sum.total() / static_cast<double>(independent);
+        const double se = std::sqrt(var / static_cast<double>(independent));
+        if (!std::isfinite(value) || !std::isfinite(se)) throw std::runtime_error("rough price estimator");
+        out->estimates[PE_C_PRICE] = pe_estimate{value, se, value - kNormal95 * se, value + kNormal95 * se, independent};
+        out->component_sum[PE_C_PRICE] = stats[k].sum.total();
+        out->component_mean[PE_C_PRICE] = stats[k].moments.mean;
+        out->component_m2[PE_C_PRICE] = stats[k].moments.m2;
+        out->sampling_variance = var;
+        out->estimator_variance = var / static_cast<double>(independent);
+        out->independent_units = independent;
+        out->evaluated_paths = paths;
+        out->time_steps = static_cast<uint32_t>(n);
+        out->path_steps = paths * n;
+        out->sobol_dimension = e.kind == PE_ENGINE_RQMC ? dim : 0;
+    }
+}
+
 }  // namespace oracle
 
 using namespace oracle;
@@ -1160,6 +1344,59 @@ int oracle_plan_coordinates(const pe_request* req, double* rows, uint32_t capaci
         }
         return static_cast<int>(n);
     } catch (const std::exception& e) { return -fail(e, 1); }
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int oracle_lifted_from_rough(double hurst, uint32_t factors, double ratio, double* weights, double* rates) {
+    try {
+        LiftedHeston m;
+        m.from_rough(hurst, factors, ratio);
+        std::copy(m.weights.begin(), m.weights.end(), weights);
+        std::copy(m.rates.begin(), m.rates.end(), rates);
+        return static_cast<int>(m.weights.size()) == static_cast<int>(factors) || hurst == 0.5 ? 0 : 1;
+    } catch (const std::exception& e) { return fail(e, 1); }
+}
+
+// One lifted Heston path (all nodes) from 2n normals: forwards and variances.
+int oracle_lifted_path(const pe_lifted_heston* m, const double* times, uint32_t nodes, double initial,
+                       const double* normals, double* forwards, double* variances) {
+    try {
+        LiftedHeston h;
+        h.v0 = m->initial_variance; h.kappa = m->mean_reversion; h.theta = m->long_run_variance;
+        h.nu = m->vol_of_vol; h.rho = m->correlation;
+        h.weights.assign(m->weights, m->weights + m->factor_count);
+        h.rates.assign(m->rates, m->rates + m->factor_count);
+        h.validate();
+        std::vector<double> f, v, factors;
+        lifted_heston_path(h, std::vector<double>(times, times + nodes), initial, normals, f, v, factors);
+        std::copy(f.begin(), f.end(), forwards);
+        std::copy(v.begin(), v.end(), variances);
+        return 0;
+    } catch (const std::exception& e) { return fail(e, 1); }
+}
+
+// RoughVolatilityPricingPlan::evaluate for `count` carriers on one path set.
+int oracle_rough_evaluate(const pe_request* const* reqs, uint32_t count, const pe_lifted_heston* model,
+                          double maximum_step, const pe_exec_policy* policy, pe_result* out,
+                          uint32_t* time_nodes_out) {
+    try {
+        RoughPlan rp;
+        compile_rough(rp, reqs, count, *model, maximum_step, policy);
+        if (time_nodes_out) *time_nodes_out = static_cast<uint32_t>(rp.grid.nodes.size());
+        evaluate_rough(rp, out);
+        return 0;
+    } catch (const EvaluationError& e) {
+        g_error = e.what();
+        out->failing_unit = e.sampling_unit;
+        return PE_ERR_NONFINITE_STATE;
+    } catch (const std::invalid_argument& e) {
+        return fail(e, PE_ERR_INVALID_ARGUMENT);
+    } catch (const std::exception& e) {
+        return fail(e, PE_ERR_UNSUPPORTED);
+    }
 }
 
 }  // extern "C"

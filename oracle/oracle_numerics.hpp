@@ -123,8 +123,14 @@ struct EvaluationError : std::runtime_error {
 template <int N>
 std::vector<Statistics> map_reduce_statistics(
     uint64_t units, uint64_t block_size, unsigned workers,
+    const std::function<void(uint64_t, double*)>& evaluate);
+
+// The same executor for a run-time number of components per unit.
+inline std::vector<Statistics> map_reduce_statistics_n(
+    uint64_t units, uint64_t block_size, unsigned workers, int width,
     const std::function<void(uint64_t, double*)>& evaluate) {
     if (block_size == 0) throw std::invalid_argument("zero reduction block size");
+    const int N = width;
     const uint64_t block_count = (units + block_size - 1) / block_size;
     std::vector<std::vector<Statistics>> partial(block_count, std::vector<Statistics>(N));
     std::atomic<uint64_t> next{0};
@@ -133,7 +139,7 @@ std::vector<Statistics> map_reduce_statistics(
     std::string failure;
     std::mutex* guard = new std::mutex();
     auto work = [&]() {
-        double values[N];
+        std::vector<double> values(N);
         for (;;) {
             const uint64_t b = next.fetch_add(1);
             if (b >= block_count || failed.load()) return;
@@ -143,7 +149,7 @@ std::vector<Statistics> map_reduce_statistics(
             for (uint64_t u = begin; u < end; ++u) {
                 for (int c = 0; c < N; ++c) values[c] = 0.0;
                 try {
-                    evaluate(u, values);
+                    evaluate(u, values.data());
                 } catch (const std::exception& e) {
                     std::lock_guard<std::mutex> lock(*guard);
                     if (u < failing_unit) { failing_unit = u; failure = e.what(); }
@@ -181,6 +187,13 @@ std::vector<Statistics> map_reduce_statistics(
         out[c].moments = fixed_tree_reduce(moments, CenteredMoment::merged);
     }
     return out;
+}
+
+template <int N>
+std::vector<Statistics> map_reduce_statistics(
+    uint64_t units, uint64_t block_size, unsigned workers,
+    const std::function<void(uint64_t, double*)>& evaluate) {
+    return map_reduce_statistics_n(units, block_size, workers, N, evaluate);
 }
 
 }  // namespace oracle

@@ -189,6 +189,33 @@ class pe_ssvi_surface(C.Structure):
 
 
 # Every symbol include/pe.h declares, with (restype, argtypes).
+class pe_lifted_heston(C.Structure):
+    _fields_ = [
+        ("initial_variance", C.c_double),
+        ("mean_reversion", C.c_double),
+        ("long_run_variance", C.c_double),
+        ("vol_of_vol", C.c_double),
+        ("correlation", C.c_double),
+        ("factor_count", C.c_uint32),
+        ("weights", _DP),
+        ("rates", _DP),
+    ]
+
+
+def lifted_heston(v0, kappa, theta, nu, rho, weights, rates):
+    """A pe_lifted_heston with its arrays kept alive on the struct."""
+    m = pe_lifted_heston()
+    w = (C.c_double * len(weights))(*weights)
+    x = (C.c_double * len(rates))(*rates)
+    m.initial_variance, m.mean_reversion, m.long_run_variance = v0, kappa, theta
+    m.vol_of_vol, m.correlation = nu, rho
+    m.factor_count = len(weights)
+    m.weights = C.cast(w, _DP)
+    m.rates = C.cast(x, _DP)
+    m._keep = (w, x)
+    return m
+
+
 PE_SYMBOLS = {
     "pe_init": (C.c_int, [C.c_int32]),
     "pe_last_error": (C.c_char_p, []),
@@ -198,6 +225,9 @@ PE_SYMBOLS = {
     "pe_plan_evaluate": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(pe_result)]),
     "pe_plans_evaluate_shared": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.c_int, C.POINTER(pe_result)]),
     "pe_batch_evaluate": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.c_int, C.POINTER(pe_result)]),
+    "pe_lifted_heston_from_rough": (C.c_int, [C.c_double, C.c_uint32, C.c_double, _DP, _DP]),
+    "pe_rough_plan_compile": (C.c_int, [C.POINTER(pe_request), C.POINTER(pe_lifted_heston), C.c_double,
+                                        C.POINTER(pe_exec_policy), C.POINTER(C.c_void_p)]),
     "pe_local_variance_from_ssvi": (
         C.c_int,
         [C.POINTER(pe_ssvi_surface), C.c_uint32, _DP, C.c_uint32, _DP, C.c_double, C.c_double, _DP,

@@ -67,6 +67,11 @@ def oracle():
     lib.oracle_jump_hit.argtypes = [C.c_int] + [C.c_double] * 4 + [D]
     lib.oracle_dividend_event.argtypes = [C.c_double] * 6 + [D]
     lib.oracle_plan_coordinates.argtypes = [C.POINTER(abi.pe_request), D, C.c_uint32, D]
+    lib.oracle_lifted_from_rough.argtypes = [C.c_double, C.c_uint32, C.c_double, D, D]
+    lib.oracle_lifted_path.argtypes = [C.POINTER(abi.pe_lifted_heston), D, C.c_uint32, C.c_double, D, D, D]
+    lib.oracle_rough_evaluate.argtypes = [C.POINTER(C.POINTER(abi.pe_request)), C.c_uint32,
+                                          C.POINTER(abi.pe_lifted_heston), C.c_double, C.POINTER(abi.pe_exec_policy),
+                                          C.POINTER(abi.pe_result), C.POINTER(C.c_uint32)]
     if lib.oracle_init(directions_path().encode()) != 0:
         raise RuntimeError(lib.oracle_last_error().decode())
     _oracle = lib
@@ -92,3 +97,19 @@ def golden(name="reference_vectors.json"):
 
 def darr(values):
     return (C.c_double * len(values))(*values)
+
+
+def oracle_rough(reqs, model, maximum_step, workers=4, block=0):
+    """Lifted Heston prices of carriers sharing one path set (one carrier: the
+    reference's RoughVolatilityPricingPlan)."""
+    from pricing_amd import abi, request
+
+    lib = oracle()
+    arr = (C.POINTER(abi.pe_request) * len(reqs))(*[r.pointer() for r in reqs])
+    out = (abi.pe_result * len(reqs))()
+    pol = request.exec_policy(workers, block)
+    nodes = C.c_uint32()
+    rc = lib.oracle_rough_evaluate(arr, len(reqs), C.byref(model), maximum_step, C.byref(pol), out, C.byref(nodes))
+    if rc != 0:
+        raise RuntimeError(f"oracle_rough_evaluate failed ({rc}): {lib.oracle_last_error().decode()}")
+    return list(out), nodes.value

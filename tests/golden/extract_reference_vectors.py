@@ -8,6 +8,8 @@ from
   fixtures/replay/{european_bs,local_volatility,path_dependence}-windows-x86_64.json
   fixtures/local-vol/reference-cases-v0.1.json  (standard_ssvi, constant_vol_dupire, affine_dividend)
   fixtures/path-dependence/reference-cases-v0.1.json  (smoothing, extrema, bridge, smoothed bridge, jumps)
+  fixtures/rough-volatility/lifted-factor-prices.json  (finite lift kernels)
+  fixtures/rough-volatility/two-step-prices.json  (lifted Heston two-step call prices)
 
 The tests read the JSON this script writes; nothing at test time reads the
 reference tree.
@@ -61,6 +63,15 @@ def main():
         pd = json.load(f)
     out["path_dependence"] = {k: pd[k] for k in ("smoothing_cases", "extrema_cases", "bridge_cases",
                                                 "smoothed_bridge_cases", "jump_cases")}
+    rough = os.path.join(ROOT, "fixtures", "rough-volatility")
+    with open(os.path.join(rough, "lifted-factor-prices.json")) as f:
+        lifted = json.load(f)
+    with open(os.path.join(rough, "two-step-prices.json")) as f:
+        two = json.load(f)
+    out["lifted_kernels"] = lifted["kernels"]
+    out["two_step_lifted_heston"] = {
+        "spot": two["spot"], "time_nodes": two["time_nodes"], "strikes": two["strikes"],
+        "case": next(c for c in two["cases"] if c["name"] == "lifted_heston")}
     with open(os.path.join(HERE, "reference_vectors.json"), "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
         f.write("\n")

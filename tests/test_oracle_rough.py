@@ -1,0 +1,112 @@
+"""Pins the oracle's lifted Heston scheme (rough-volatility adapter).
+
+* Factor quadrature: LiftedHeston::from_rough weights and rates reproduce the
+  reference's finite lift kernels sum_i w_i (1 + x_i dt)^-l on five lags for
+  H = 0.1, 0.3, maturities 0.25 and 1, 8..256 factors
+  (fixtures/rough-volatility/lifted-factor-prices.json).
+* Scheme: Monte Carlo prices of the two-step lifted Heston log-Euler scheme
+  against the reference's independent quadrature of the same finite scheme
+  (fixtures/rough-volatility/two-step-prices.json, strikes 90/100/110).
+* Black-Scholes limit: with nu = kappa = 0 the variance stays v0 and the asset
+  is exact GBM, so European prices match the closed form and carry the curve
+  scale q/p.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy.stats import norm
+
+from _support import darr, golden, oracle, oracle_rough
+from pricing_amd import abi, request
+
+G = golden()
+
+
+def from_rough(hurst, n, ratio):
+    w = (C_double := __import__("ctypes").c_double * n)()
+    x = C_double()
+    assert oracle().oracle_lifted_from_rough(hurst, n, ratio, w, x) == 0, oracle().oracle_last_error()
+    return list(w), list(x)
+
+
+@pytest.mark.parametrize("kernel", G["lifted_kernels"], ids=lambda k: f"H{k['hurst']}-T{k['maturity']}")
+def test_from_rough_reproduces_finite_lift_kernels(kernel):
+    dt = kernel["maturity"] / kernel["steps"]
+    for fk in kernel["finite_kernels"]:
+        w, x = from_rough(kernel["hurst"], fk["factors"], fk["ratio"])
+        for lag, want in zip(kernel["lags"], fk["values"]):
+            got = sum(wi * (1.0 + xi * dt) ** (-lag) for wi, xi in zip(w, x))
+            assert abs(got - want) <= 1e-13 * abs(want), (fk["factors"], lag, got, want)
+
+
+def test_from_rough_heston_limit_is_one_unit_factor():
+    w, x = from_rough(0.5, 4, 2.0)
+    assert w[0] == 1.0 and x[0] == 0.0
+
+
+def _carrier(expiry_day, strike, rate=0.0, q=0.0, units=1 << 14, scrambles=0, antithetic=True, bridge=False,
+             seed=0xC0FFEE):
+    req = request.Request().set_market(100.0, [0.0, 1.0], [1.0, math.exp(-rate)], [0.0, 1.0], [1.0, math.exp(-q)])
+    req.set_black_scholes(0.2)
+    req.set_product(abi.PE_PRODUCT_EUROPEAN, abi.PE_CALL, expiry_day, strike)
+    if scrambles:
+        req.set_rqmc(seed, units, scrambles, antithetic=antithetic, brownian_bridge=bridge)
+    else:
+        req.set_pseudo(seed, units, antithetic=antithetic, brownian_bridge=bridge)
+    req.set_risk()
+    return req
+
+
+def test_two_step_prices_match_reference_quadrature():
+    t = G["two_step_lifted_heston"]
+    p = t["case"]["parameters"]
+    model = abi.lifted_heston(p["initial_variance"], p["mean_reversion"], p["long_run_variance"], p["vol_of_vol"],
+                              p["correlation"], p["weights"], p["rates"])
+    reqs = [_carrier(365, k, units=1 << 14, scrambles=16) for k in t["strikes"]]
+    res, nodes = oracle_rough(reqs, model, 0.5, workers=8)
+    assert nodes == 3  # time nodes [0, 0.5, 1]
+    for r, want in zip(res, t["case"]["call_prices"]):
+        est = r.estimates[0]
+        assert abs(est.value - want) <= 4.0 * est.standard_error + 1e-4, (est.value, want, est.standard_error)
+
+
+def test_black_scholes_limit_matches_closed_form():
+    v0, rate, q = 0.04, 0.03, 0.01
+    model = abi.lifted_heston(v0, 0.0, 0.0, 0.0, -0.5, [0.5, 1.5], [0.2, 3.0])
+    reqs = [_carrier(365, k, rate=rate, q=q, units=1 << 15) for k in (80.0, 100.0, 125.0)]
+    res, nodes = oracle_rough(reqs, model, 1.0 / 52, workers=8)
+    sig = math.sqrt(v0)
+    for k, r in zip((80.0, 100.0, 125.0), res):
+        f = 100.0 * math.exp(rate - q)
+        d1 = (math.log(f / k) + 0.5 * v0) / sig
+        closed = math.exp(-rate) * (f * norm.cdf(d1) - k * norm.cdf(d1 - sig))
+        est = r.estimates[0]
+        assert abs(est.value - closed) <= 4.0 * est.standard_error, (k, est.value, closed, est.standard_error)
+
+
+def test_single_carrier_shared_equals_single():
+    model = abi.lifted_heston(0.02, 0.3, 0.02, 0.3, -0.7, *from_rough(0.1, 20, 2.5))
+    a = _carrier(182, 100.0, units=2048)
+    b = _carrier(365, 95.0, units=2048)
+    single, _ = oracle_rough([a], model, 2.0 / 256, workers=4)
+    again, _ = oracle_rough([a], model, 2.0 / 256, workers=1)
+    assert single[0].estimates[0].value == again[0].estimates[0].value
+    shared, nodes = oracle_rough([a, b], model, 2.0 / 256, workers=4)
+    # b's events extend the grid beyond a's expiry: a is priced on its own nodes
+    assert abs(shared[0].estimates[0].value - single[0].estimates[0].value) <= 4.0 * single[0].estimates[0].standard_error
+
+
+def test_bridge_changes_paths_not_the_law():
+    v0 = 0.04
+    model = abi.lifted_heston(v0, 0.0, 0.0, 0.0, 0.3, [1.0], [0.0])
+    vals = []
+    for bridge in (False, True):
+        req = _carrier(365, 100.0, units=1 << 12, scrambles=8, bridge=bridge)
+        r, _ = oracle_rough([req], model, 1.0 / 16, workers=8)
+        vals.append(r[0].estimates[0])
+    sig = math.sqrt(v0)
+    closed = 100.0 * (norm.cdf(0.5 * sig) - norm.cdf(-0.5 * sig))
+    assert vals[0].value != vals[1].value
+    for e in vals:
+        assert abs(e.value - closed) <= 4.0 * e.standard_error + 1e-6
