@@ -349,6 +349,173 @@ void evaluate_bs_batch(std::vector<HostPlan*>& plans, pe_backend backend, pe_res
     g_kernel_launches = launches;
 }
 
+// ---------------------------------------------------------- lifted Heston --
+// One simulated path set for every plan: the union of the plans' events is
+// the grid (one plan: the reference's own grid).  The path kernel stores the
+// physical spot at each payoff's expiry node; the payoff kernel reduces.
+void evaluate_rough(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* outs) {
+    const HostPlan& p0 = *plans[0];
+    const pe_engine& e = p0.req.engine;
+    std::vector<double> events;
+    for (const HostPlan* q : plans) {
+        if (!q->rough) throw PlanError(PE_ERR_INVALID_ARGUMENT, "mixing rough and non-rough plans");
+        const pe_engine& f = q->req.engine;
+        if (f.kind != e.kind || f.seed != e.seed || f.units != e.units || f.scrambles != e.scrambles ||
+            f.antithetic != e.antithetic || f.brownian_bridge != e.brownian_bridge || q->spot != p0.spot ||
+            q->rough_max_step != p0.rough_max_step || q->lifted.weights != p0.lifted.weights ||
+            q->lifted.rates != p0.lifted.rates || q->lifted.v0 != p0.lifted.v0 ||
+            q->lifted.kappa != p0.lifted.kappa || q->lifted.theta != p0.lifted.theta ||
+            q->lifted.nu != p0.lifted.nu || q->lifted.rho != p0.lifted.rho)
+            throw PlanError(PE_ERR_INVALID_ARGUMENT, "shared rough plans need one model, engine, spot and step");
+        events.insert(events.end(), q->rough_events.begin(), q->rough_events.end());
+    }
+    if (e.brownian_bridge) throw PlanError(PE_ERR_UNSUPPORTED, "Brownian-bridge ordering of lifted Heston paths on the device");
+    const std::vector<double> nodes = compile_time_grid(events, p0.rough_max_step);
+    const uint32_t n = static_cast<uint32_t>(nodes.size() - 1);
+    if (n > 65536) throw PlanError(PE_ERR_SIZE_LIMIT, "rough time grid resource limit");
+    const HostLifted& h = p0.lifted;
+    const uint32_t nf = static_cast<uint32_t>(h.weights.size());
+    // physical spot per unit of F (HullWhiteDividendPlan node scale without dividends)
+    auto scale_at = [&](double t) {
+        const double pd = std::exp(p0.discount_curve.log_discount(t));
+        const double qd = std::exp(p0.dividend_curve.log_discount(t));
+        return 1.0 * p0.spot / p0.spot * qd / pd;
+    };
+    std::vector<RoughStep> steps(n);
+    std::vector<double> den(static_cast<size_t>(n) * nf);
+    std::vector<float> rden(den.size());
+    for (uint32_t j = 0; j < n; ++j) {
+        const double dt = nodes[j + 1] - nodes[j];
+        steps[j] = RoughStep{dt, std::sqrt(dt), scale_at(nodes[j + 1]), 0.0};
+        for (uint32_t i = 0; i < nf; ++i) {
+            const double d = 1.0 + h.rates[i] * dt;
+            if (!std::isfinite(d)) throw PlanError(PE_ERR_INVALID_ARGUMENT, "lifted time scale overflow");
+            den[static_cast<size_t>(j) * nf + i] = d;
+            rden[static_cast<size_t>(j) * nf + i] = static_cast<float>(1.0 / d);
+        }
+    }
+    // store slots: the distinct expiry nodes of the plans
+    std::vector<uint32_t> store_nodes;
+    std::vector<uint32_t> plan_node(plans.size());
+    for (size_t k = 0; k < plans.size(); ++k) {
+        const auto it = std::lower_bound(nodes.begin(), nodes.end(), plans[k]->expiry);
+        if (it == nodes.end() || *it != plans[k]->expiry) throw PlanError(PE_ERR_INVALID_ARGUMENT, "missing expiry node");
+        plan_node[k] = static_cast<uint32_t>(it - nodes.begin());
+        store_nodes.push_back(plan_node[k]);
+    }
+    std::sort(store_nodes.begin(), store_nodes.end());
+    store_nodes.erase(std::unique(store_nodes.begin(), store_nodes.end()), store_nodes.end());
+    const bool rqmc = e.kind == PE_ENGINE_RQMC;
+    const uint32_t scrambles = rqmc ? e.scrambles : 1u;
+    const int passes = e.antithetic ? 2 : 1;
+    const uint64_t total_paths = e.units * scrambles * passes;
+    const bool fp64 = backend == PE_BACKEND_GPU_FP64;
+    DeviceBuffer d_steps, d_den, d_rden, d_store_nodes, d_store, d_sobol, d_err, d_products, d_items, d_partials;
+    d_steps.upload(steps);
+    d_den.upload(den);
+    d_rden.upload(rden);
+    d_store_nodes.upload(store_nodes);
+    d_store.reserve(total_paths * store_nodes.size() * (fp64 ? sizeof(double) : sizeof(float)));
+    if (rqmc) d_sobol.upload(build_sobol_table(e.seed, e.scrambles, 2u * n));
+    RoughArgs a{};
+    a.steps = d_steps.as<RoughStep>();
+    a.denominators = d_den.as<double>();
+    a.reciprocals = d_rden.as<float>();
+    a.sobol = d_sobol.as<uint32_t>();
+    a.store_nodes = d_store_nodes.as<uint32_t>();
+    a.store = d_store.ptr;
+    a.seed = e.seed;
+    a.units = e.units;
+    a.total_paths = total_paths;
+    a.spot = p0.spot;
+    a.scale0 = scale_at(0.0);
+    a.v0 = h.v0;
+    a.kappa = h.kappa;
+    a.theta = h.theta;
+    a.nu = h.nu;
+    a.rho = h.rho;
+    a.rho_bar = std::sqrt(std::max(1.0 - h.rho * h.rho, 0.0));
+    for (uint32_t i = 0; i < nf; ++i) {
+        a.weights[i] = h.weights[i];
+        a.weights32[i] = static_cast<float>(h.weights[i]);
+    }
+    a.v0_32 = static_cast<float>(h.v0);
+    a.n_steps = n;
+    a.n_factors = nf;
+    a.n_store = static_cast<uint32_t>(store_nodes.size());
+    a.rqmc = rqmc ? 1u : 0u;
+    a.antithetic = e.antithetic ? 1u : 0u;
+    // payoff work: (product, scramble, 4096-unit chunk)
+    std::vector<RoughProduct> products(plans.size());
+    std::vector<RoughItem> items;
+    const uint32_t chunks = static_cast<uint32_t>((e.units + 4095) / 4096);
+    for (size_t k = 0; k < plans.size(); ++k) {
+        const ProductDesc& d = plans[k]->product;
+        RoughProduct& rp = products[k];
+        rp.kind = d.kind;
+        rp.call = d.call;
+        rp.payout_asset = d.payout_asset;
+        rp.slot = static_cast<uint32_t>(std::lower_bound(store_nodes.begin(), store_nodes.end(), plan_node[k]) -
+                                        store_nodes.begin());
+        rp.strike = d.strike;
+        rp.notional = d.notional;
+        rp.payout = d.payout;
+        rp.discount = d.discount;
+        for (uint32_t sc = 0; sc < scrambles; ++sc)
+            for (uint32_t c = 0; c < chunks; ++c) {
+                RoughItem it{};
+                it.product = static_cast<uint32_t>(k);
+                it.scramble = sc;
+                it.unit_begin = static_cast<uint64_t>(c) * 4096;
+                it.unit_end = std::min<uint64_t>(it.unit_begin + 4096, e.units);
+                it.partial = static_cast<uint32_t>((k * scrambles + sc) * chunks + c);
+                items.push_back(it);
+            }
+    }
+    d_products.upload(products);
+    d_items.upload(items);
+    d_partials.reserve(items.size() * sizeof(BlockStat));
+    RoughPayoffArgs pa{};
+    pa.items = d_items.as<RoughItem>();
+    pa.products = d_products.as<RoughProduct>();
+    pa.store = d_store.ptr;
+    pa.partials = d_partials.ptr;
+    pa.units = e.units;
+    pa.total_paths = total_paths;
+    pa.antithetic = a.antithetic;
+    d_err.reserve(sizeof(unsigned long long));
+    const unsigned long long none = ~0ull;
+    check(hipMemcpyAsync(d_err.ptr, &none, sizeof(none), hipMemcpyHostToDevice, g_stream), "error word");
+    const uint64_t blocks = (e.units + kBlockThreads - 1) / kBlockThreads;
+    const dim3 grid(static_cast<uint32_t>(std::min<uint64_t>(blocks, 1u << 14)), scrambles, 1u);
+    const float ms = timed([&] {
+        check(fp64 ? launch_rough_f64(a, pa, grid, static_cast<uint32_t>(items.size()),
+                                      static_cast<unsigned long long*>(d_err.ptr), g_stream)
+                   : launch_rough_f32(a, pa, grid, static_cast<uint32_t>(items.size()),
+                                      static_cast<unsigned long long*>(d_err.ptr), g_stream),
+              "rough kernel launch");
+    });
+    std::vector<BlockStat> host(items.size());
+    check(hipMemcpy(host.data(), d_partials.ptr, host.size() * sizeof(BlockStat), hipMemcpyDeviceToHost), "partials");
+    unsigned long long err = none;
+    check(hipMemcpy(&err, d_err.ptr, sizeof(err), hipMemcpyDeviceToHost), "error word");
+    if (err != none)
+        throw PlanError(PE_ERR_NONFINITE_STATE, "sampling unit " + std::to_string(err) + ": non-finite lifted Heston path");
+    for (size_t k = 0; k < plans.size(); ++k) {
+        std::vector<BlockStat> mine(host.begin() + static_cast<long>(k * scrambles * chunks),
+                                    host.begin() + static_cast<long>((k + 1) * scrambles * chunks));
+        pe_result* out = outs + k;
+        std::memset(out, 0, sizeof(*out));
+        assemble(*plans[k], mine, chunks, scrambles, 1, 0, 1, out);
+        out->time_steps = n;
+        out->path_steps = out->evaluated_paths * n;
+        out->sobol_dimension = rqmc ? 2u * n : 0u;
+        out->device_ms = ms;
+    }
+    g_kernel_ms = ms;
+    g_kernel_launches = 2;
+}
+
 void upload_lv_tables(HostPlan& p) {
     if (p.d_steps.ptr) return;
     p.d_steps.upload(p.steps);
@@ -627,7 +794,10 @@ pe_status pe_plan_evaluate(const pe_plan* plan, pe_backend backend, pe_result* o
     return guarded([&] {
         ensure_init();
         HostPlan& p = const_cast<pe_plan*>(plan)->host;
-        if (p.lv) {
+        if (p.rough) {
+            std::vector<HostPlan*> plans{&p};
+            evaluate_rough(plans, backend, out);
+        } else if (p.lv) {
             std::vector<HostPlan*> plans{&p};
             evaluate_lv(plans, backend, out);
         } else {
@@ -647,8 +817,30 @@ pe_status pe_plans_evaluate_shared(const pe_plan* const* plans, uint32_t count, 
             if (!plans[i]) throw PlanError(PE_ERR_INVALID_ARGUMENT, "null plan");
             hp.push_back(&const_cast<pe_plan*>(plans[i])->host);
         }
-        evaluate_lv(hp, backend, out);
+        if (hp[0]->rough) evaluate_rough(hp, backend, out);
+        else evaluate_lv(hp, backend, out);
     });
+}
+
+pe_status pe_lifted_heston_from_rough(double hurst, uint32_t factors, double ratio, double* weights_out,
+                                      double* rates_out) {
+    if (!weights_out || !rates_out) return fail(PE_ERR_INVALID_ARGUMENT, "null argument");
+    return guarded([&] {
+        std::vector<double> w, x;
+        lifted_from_rough(hurst, factors, ratio, w, x);
+        std::copy(w.begin(), w.end(), weights_out);
+        std::copy(x.begin(), x.end(), rates_out);
+    });
+}
+
+pe_status pe_rough_plan_compile(const pe_request* carrier, const pe_lifted_heston* model, double maximum_step,
+                                const pe_exec_policy* policy, pe_plan** out) {
+    if (!carrier || !model || !out) return fail(PE_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    auto plan = std::make_unique<pe_plan>();
+    const pe_status st = guarded([&] { compile_rough_plan(plan->host, *carrier, *model, maximum_step, policy); });
+    if (st == PE_OK) *out = plan.release();
+    return st;
 }
 
 pe_status pe_batch_evaluate(const pe_plan* const* plans, uint32_t count, pe_backend backend, pe_result* results) {

@@ -2,12 +2,18 @@
 // lookups; FP64 path accumulators and statistics.
 #include "pe_kernels.inc"
 #include "pe_bs.inc"
+#include "pe_rough.inc"
 #include "pe_plan.hpp"
 
 namespace pe {
 
 hipError_t launch_bs_batch_f32(const BsBatchArgs& a, bool risk, bool cont, hipStream_t s) {
     return launch_bs_batch<float>(a, risk, cont, s);
+}
+
+hipError_t launch_rough_f32(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 grid, uint32_t payoff_items,
+                            unsigned long long* error, hipStream_t s) {
+    return launch_rough<float>(a, pa, grid, payoff_items, error, s);
 }
 
 hipError_t launch_lv_f32(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s) {

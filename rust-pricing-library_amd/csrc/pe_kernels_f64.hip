@@ -2,12 +2,18 @@
 // keeps the reference's rounding sequence (no fused multiply-add).
 #include "pe_kernels.inc"
 #include "pe_bs.inc"
+#include "pe_rough.inc"
 #include "pe_plan.hpp"
 
 namespace pe {
 
 hipError_t launch_bs_batch_f64(const BsBatchArgs& a, bool risk, bool cont, hipStream_t s) {
     return launch_bs_batch<double>(a, risk, cont, s);
+}
+
+hipError_t launch_rough_f64(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 grid, uint32_t payoff_items,
+                            unsigned long long* error, hipStream_t s) {
+    return launch_rough<double>(a, pa, grid, payoff_items, error, s);
 }
 
 hipError_t launch_lv_f64(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s) {

@@ -412,6 +412,111 @@ void compile_bs_trade(HostPlan& p, double monitoring_end) {
     t.n_variants = nv;
 }
 
+// ------------------------------------------------------------ time grid --
+std::vector<double> compile_time_grid(std::vector<double> events, double maximum_step) {
+    if (!std::isfinite(maximum_step) || maximum_step <= 0.0) invalid("maximum step must be positive");
+    std::sort(events.begin(), events.end());
+    events.erase(std::unique(events.begin(), events.end()), events.end());
+    if (events.empty() || events[0] != 0.0) events.insert(events.begin(), 0.0);
+    std::vector<double> nodes(1, 0.0);
+    for (size_t i = 0; i + 1 < events.size(); ++i) {
+        const double l = events[i], rr = events[i + 1];
+        const double interval = rr - l;
+        const double raw = std::ceil(interval / maximum_step);
+        if (!std::isfinite(raw) || raw > 1e9) throw PlanError(PE_ERR_SIZE_LIMIT, "time grid step count");
+        const size_t steps = std::max<size_t>(1, static_cast<size_t>(raw));
+        const double sub = interval / static_cast<double>(steps);
+        for (size_t k = 1; k < steps; ++k) nodes.push_back(l + sub * static_cast<double>(k));
+        nodes.push_back(rr);
+    }
+    return nodes;
+}
+
+// ------------------------------------------------------- lifted Heston --
+// gamma_half_to_two (crates/pricing-numerics/src/fractional.rs:1-30): Lanczos
+// with g = 7 and Godfrey's nine published coefficients, on [1/2, 2].
+static double lanczos_gamma(double x) {
+    if (!std::isfinite(x) || !(x >= 0.5 && x <= 2.0)) invalid("gamma argument outside [1/2, 2]");
+    if (x == 1.0 || x == 2.0) return 1.0;
+    static const double c[8] = {676.5203681218851,     -1259.1392167224028,   771.3234287776531,
+                                -176.6150291621406,    12.507343278686905,    -0.13857109526572012,
+                                9.984369578019572e-6, 1.5056327351493116e-7};
+    const double z = x - 1.0;
+    double sum = 0.9999999999998099;
+    for (int i = 0; i < 8; ++i) sum += c[i] / (z + static_cast<double>(i) + 1.0);
+    const double t = z + 7.5;
+    return std::sqrt(2.0 * M_PI) * std::pow(t, z + 0.5) * std::exp(-t) * sum;
+}
+
+// LiftedHeston::from_rough (crates/pricing/src/models/rough_volatility.rs:564-606).
+void lifted_from_rough(double hurst, uint32_t factors, double ratio, std::vector<double>& weights,
+                       std::vector<double>& rates) {
+    if (factors == 0 || factors > 256 || !std::isfinite(ratio) || ratio <= 1.0) invalid("lifted Heston quadrature");
+    if (!std::isfinite(hurst) || hurst <= 0.0 || hurst > 0.5) invalid("Hurst index must be in (0, 1/2]");
+    weights.clear();
+    rates.clear();
+    if (hurst == 0.5) {
+        weights.push_back(1.0);
+        rates.push_back(0.0);
+        return;
+    }
+    const double alpha = hurst + 0.5;
+    const double p = 1.0 - alpha;
+    const double log_ratio = std::log(ratio);
+    const double denominator = lanczos_gamma(alpha) * lanczos_gamma(2.0 - alpha);
+    const double mass = std::expm1(p * log_ratio) / denominator;
+    const double rate = p / (1.0 + p) * std::expm1((1.0 + p) * log_ratio) / std::expm1(p * log_ratio);
+    for (uint32_t j = 0; j < factors; ++j) {
+        const double log_lower = (static_cast<double>(j) - static_cast<double>(factors) / 2.0) * log_ratio;
+        weights.push_back(mass * std::exp(p * log_lower));
+        rates.push_back(rate * std::exp(log_lower));
+    }
+}
+
+// RoughVolatilityPricingPlan::compile (crates/pricing/src/engine/risk/rough_volatility.rs:84-190).
+void compile_rough_plan(HostPlan& p, const pe_request& carrier, const pe_lifted_heston& m, double maximum_step,
+                        const pe_exec_policy* policy) {
+    if (carrier.model.kind != PE_MODEL_BLACK_SCHOLES) invalid("rough extension requires a Black-Scholes request carrier");
+    if (carrier.risk.delta || carrier.risk.vega || carrier.risk.gamma) unsupported("rough pricing is price-only");
+    if (carrier.market.dividend_count) unsupported("discrete dividends under lifted Heston");
+    if (carrier.product.kind == PE_PRODUCT_BARRIER && carrier.product.continuous)
+        unsupported("continuous barriers in the rough adapter");
+    if (carrier.product.kind != PE_PRODUCT_EUROPEAN && carrier.product.kind != PE_PRODUCT_DIGITAL)
+        unsupported("lifted Heston on the device prices terminal payoffs (European, Digital)");
+    compile_host_plan(p, carrier, policy);
+    HostLifted& h = p.lifted;
+    h.v0 = m.initial_variance;
+    h.kappa = m.mean_reversion;
+    h.theta = m.long_run_variance;
+    h.nu = m.vol_of_vol;
+    h.rho = m.correlation;
+    for (double v : {h.v0, h.kappa, h.theta, h.nu})
+        if (!std::isfinite(v) || v < 0.0) invalid("lifted Heston parameters must be finite and non-negative");
+    if (!std::isfinite(h.rho) || h.rho < -1.0 || h.rho > 1.0) invalid("lifted Heston correlation");
+    if (m.factor_count == 0 || m.factor_count > 256 || !m.weights || !m.rates) invalid("lifted Heston factor count");
+    if (m.factor_count > static_cast<uint32_t>(kMaxRoughFactors))
+        throw PlanError(PE_ERR_SIZE_LIMIT, "the device keeps at most 32 lifted Heston factors");
+    h.weights.assign(m.weights, m.weights + m.factor_count);
+    h.rates.assign(m.rates, m.rates + m.factor_count);
+    for (size_t i = 0; i < h.weights.size(); ++i) {
+        if (!std::isfinite(h.weights[i]) || h.weights[i] <= 0.0) invalid("lifted Heston weight");
+        if (!std::isfinite(h.rates[i]) || h.rates[i] < 0.0) invalid("lifted Heston rate");
+    }
+    if (!std::isfinite(p.expiry) || p.expiry <= 0.0 || !std::isfinite(maximum_step) || maximum_step <= 0.0)
+        invalid("rough positive horizon and step");
+    p.rough = true;
+    p.rough_max_step = maximum_step;
+    p.rough_events = p.obs_times;
+    p.rough_events.push_back(0.0);
+    p.rough_events.push_back(p.expiry);
+    std::sort(p.rough_events.begin(), p.rough_events.end());
+    p.rough_events.erase(std::unique(p.rough_events.begin(), p.rough_events.end()), p.rough_events.end());
+    double estimated = 0.0;
+    for (size_t i = 0; i + 1 < p.rough_events.size(); ++i)
+        estimated += std::ceil((p.rough_events[i + 1] - p.rough_events[i]) / maximum_step);
+    if (!std::isfinite(estimated) || estimated > 65536.0) throw PlanError(PE_ERR_SIZE_LIMIT, "rough time grid resource limit");
+}
+
 // ------------------------------------------------------------------ plan --
 void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* policy) {
     static std::atomic<uint64_t> next_id{1};
@@ -586,20 +691,7 @@ void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* p
         // LocalVolTimeGrid::compile: union of grid nodes and events, ceil(dt / max) substeps.
         std::vector<double> events = p.lv_t;
         events.insert(events.end(), p.obs_times.begin(), p.obs_times.end());
-        std::sort(events.begin(), events.end());
-        events.erase(std::unique(events.begin(), events.end()), events.end());
-        if (events[0] != 0.0) events.insert(events.begin(), 0.0);
-        p.nodes.assign(1, 0.0);
-        for (size_t i = 0; i + 1 < events.size(); ++i) {
-            const double l = events[i], rr = events[i + 1];
-            const double interval = rr - l;
-            const double raw = std::ceil(interval / max_step);
-            if (!std::isfinite(raw) || raw > 1e9) throw PlanError(PE_ERR_SIZE_LIMIT, "local-vol step count");
-            const size_t steps = std::max<size_t>(1, static_cast<size_t>(raw));
-            const double sub = interval / static_cast<double>(steps);
-            for (size_t k = 1; k < steps; ++k) p.nodes.push_back(l + sub * static_cast<double>(k));
-            p.nodes.push_back(rr);
-        }
+        p.nodes = compile_time_grid(events, max_step);
         for (double t : p.nodes) p.node_forwards.push_back(p.forward_at(t));
         p.obs_nodes.clear();
         for (double t : p.obs_times) {

@@ -106,6 +106,12 @@ struct HostDividends {
     HostCoord after_time(double t) const;
 };
 
+// Lifted Heston model of a rough-volatility plan.
+struct HostLifted {
+    double v0 = 0, kappa = 0, theta = 0, nu = 0, rho = 0;
+    std::vector<double> weights, rates;
+};
+
 struct BsLaunch;
 struct BsLaunchDeleter {
     void operator()(BsLaunch* l) const;
@@ -152,6 +158,13 @@ struct HostPlan {
     // RQMC
     uint32_t sobol_dims = 0;
 
+    // rough-volatility adapter (lifted Heston); the carrier's BS fields above
+    // supply the payoff, discount and observation times
+    bool rough = false;
+    HostLifted lifted;
+    double rough_max_step = 0.0;
+    std::vector<double> rough_events;
+
     // BS/Black-76 trade descriptor and observation table for the batch kernel
     BsTrade trade{};
     std::vector<BsObs> bs_obs;
@@ -163,6 +176,12 @@ struct HostPlan {
 };
 
 void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* policy);
+void compile_rough_plan(HostPlan& p, const pe_request& carrier, const pe_lifted_heston& m, double maximum_step,
+                        const pe_exec_policy* policy);
+void lifted_from_rough(double hurst, uint32_t factors, double ratio, std::vector<double>& weights,
+                       std::vector<double>& rates);
+// LocalVolTimeGrid::compile: sorted unique events, ceil(gap / maximum_step) equal substeps.
+std::vector<double> compile_time_grid(std::vector<double> events, double maximum_step);
 std::vector<uint32_t> build_sobol_table(uint64_t seed, uint32_t scrambles, uint32_t dims);
 void compile_bridge_program(const std::vector<double>& times, std::vector<BridgeOp>& program,
                             std::vector<double>& inv_sqrt_dt, uint32_t& depth);
@@ -173,6 +192,10 @@ constexpr int32_t kNoDay = INT32_MIN;
 // Launchers (pe_kernels_f64.hip / pe_kernels_f32.hip).
 hipError_t launch_bs_batch_f64(const BsBatchArgs& a, bool risk, bool cont, hipStream_t s);
 hipError_t launch_bs_batch_f32(const BsBatchArgs& a, bool risk, bool cont, hipStream_t s);
+hipError_t launch_rough_f64(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 grid, uint32_t payoff_items,
+                            unsigned long long* error, hipStream_t s);
+hipError_t launch_rough_f32(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 grid, uint32_t payoff_items,
+                            unsigned long long* error, hipStream_t s);
 hipError_t launch_lv_f64(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s);
 hipError_t launch_lv_f32(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s);
 hipError_t launch_dupire(const double* theta_t, const double* theta_v, const double* theta_d, uint32_t nk,

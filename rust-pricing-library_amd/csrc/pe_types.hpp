@@ -183,6 +183,54 @@ struct BsBatchArgs {
     uint32_t components;       // statistics per item (1, 10 or 18)
 };
 
+// ------------------------------------------------------- lifted Heston --
+constexpr int kMaxRoughFactors = 32;
+
+struct RoughStep {       // per time step of the shared grid
+    double dt, sqrt_dt;
+    double scale_next;   // physical spot per unit of F at the step's right node
+    double pad;
+};
+
+struct RoughArgs {
+    const RoughStep* steps;
+    const double* denominators;   // [step][factor]: 1 + x_i dt (FP64 mode divides by it)
+    const float* reciprocals;     // [step][factor]: 1 / (1 + x_i dt) (mixed mode)
+    const uint32_t* sobol;        // [scramble][dim][33], dims = 2 * steps
+    const uint32_t* store_nodes;  // ascending grid nodes whose spot is stored
+    void* store;                  // [slot][path]: double (FP64 mode) or float
+    uint64_t seed, units, total_paths;
+    double spot, scale0;
+    double v0, kappa, theta, nu, rho, rho_bar;
+    double weights[kMaxRoughFactors];
+    float weights32[kMaxRoughFactors];
+    float v0_32;
+    uint32_t n_steps, n_factors, n_store, rqmc, antithetic;
+};
+
+struct RoughProduct {    // terminal payoff read from one store slot
+    int32_t kind;        // kEuropean or kDigital
+    int32_t call, payout_asset;
+    uint32_t slot;
+    double strike, notional, payout, discount;
+};
+
+struct RoughItem {
+    uint32_t product, scramble;
+    uint64_t unit_begin, unit_end;
+    uint32_t partial, pad;
+};
+
+struct RoughPayoffArgs {
+    const RoughItem* items;
+    const RoughProduct* products;
+    const void* store;
+    void* partials;
+    uint64_t units, total_paths;
+    uint32_t antithetic;
+    uint32_t pad;
+};
+
 // Kernel arguments of the local-vol Log-Euler kernel.
 struct LvArgs {
     const LvStep* steps;

@@ -84,6 +84,32 @@ class PricingPlan:
             pass
 
 
+class RoughPlan(PricingPlan):
+    """A lifted Heston plan on a Black-Scholes request carrier
+    (RoughVolatilityPricingPlan::compile)."""
+
+    @classmethod
+    def compile(cls, carrier, model, maximum_step, policy=None):
+        from . import request as _request
+
+        lib = load()
+        pol = policy if policy is not None else _request.exec_policy()
+        out = C.c_void_p()
+        _check(lib.pe_rough_plan_compile(carrier.pointer(), C.byref(model), maximum_step, C.byref(pol),
+                                         C.byref(out)))
+        plan = cls(out.value, carrier)
+        plan._model = model
+        return plan
+
+
+def lifted_from_rough(hurst, factors, ratio):
+    w = (C.c_double * factors)()
+    x = (C.c_double * factors)()
+    _check(load().pe_lifted_heston_from_rough(hurst, factors, ratio, w, x))
+    n = 1 if hurst == 0.5 else factors
+    return list(w)[:n], list(x)[:n]
+
+
 def evaluate_shared(plans, backend="fp64"):
     """Evaluates plans sharing one simulated path set in one launch."""
     n = len(plans)

@@ -175,3 +175,41 @@ def batch_trades(count=4096, seed=0x00C0FFEE, min_log2=12.0, max_log2=18.0, risk
         req.trade_id = tid
         out.append(req)
     return out
+
+
+# ----------------------------------------------------- lifted Heston smile --
+ROUGH = {"hurst": 0.1, "v0": 0.02, "theta": 0.02, "kappa": 0.3, "nu": 0.3, "rho": -0.7, "factors": 20,
+         "ratio": 2.5}
+SMILE_MATURITIES = (0.1, 0.25, 0.5, 0.75, 1.0, 1.25, 1.5, 2.0)
+
+
+def lifted_heston_model(weights_rates):
+    """RoughHeston{H=0.1, v0=0.02, kappa=0.3, theta=0.02, nu=0.3, rho=-0.7}
+    lifted with 20 geometric factors of ratio 2.5; weights_rates =
+    (weights, rates) from LiftedHeston::from_rough."""
+    w, x = weights_rates
+    r = ROUGH
+    return abi.lifted_heston(r["v0"], r["kappa"], r["theta"], r["nu"], r["rho"], w, x)
+
+
+def rough_smile(points=1 << 16, scrambles=16, strikes=21, maturities=SMILE_MATURITIES, rate=0.0, seed=0x5EED5EED,
+                antithetic=False):
+    """European call carriers of the lifted Heston smile: 21 strikes
+    0.8..1.2 S0 x 8 maturities (round(365 m) days), RQMC points x scrambles
+    (pseudo-MC units when scrambles = 0)."""
+    out = []
+    for m in maturities:
+        day = int(round(365 * m))
+        for i in range(strikes):
+            k = 100.0 * (0.8 + 0.4 * i / (strikes - 1))
+            req = request.Request()
+            req.set_market(100.0, [0.0, 1.0], [1.0, math.exp(-rate)], [0.0, 1.0], [1.0, 1.0])
+            req.set_black_scholes(0.2)
+            req.set_product(abi.PE_PRODUCT_EUROPEAN, abi.PE_CALL, day, k)
+            if scrambles:
+                req.set_rqmc(seed, points, scrambles, antithetic=antithetic)
+            else:
+                req.set_pseudo(seed, points, antithetic=antithetic)
+            req.set_risk()
+            out.append(req)
+    return out
