@@ -353,8 +353,68 @@ void evaluate_bs_batch(std::vector<HostPlan*>& plans, pe_backend backend, pe_res
 // One simulated path set for every plan: the union of the plans' events is
 // the grid (one plan: the reference's own grid).  The path kernel stores the
 // physical spot at each payoff's expiry node; the payoff kernel reduces.
+}  // namespace
+
+struct RoughLaunch {
+    std::vector<uint64_t> key;
+    int backend = 0;
+    RoughArgs a{};
+    RoughPayoffArgs pa{};
+    dim3 grid;
+    uint32_t n = 0, chunks = 0, scrambles = 1;
+    size_t items = 0;
+    bool rqmc = false;
+    DeviceBuffer d_steps, d_den, d_rden, d_store_nodes, d_store, d_sobol, d_err, d_products, d_items, d_partials;
+    std::vector<BlockStat> host;
+};
+
+void RoughLaunchDeleter::operator()(RoughLaunch* l) const { delete l; }
+
+namespace {
+
+void prepare_rough(std::vector<HostPlan*>& plans, pe_backend backend);
+
 void evaluate_rough(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* outs) {
-    const HostPlan& p0 = *plans[0];
+    HostPlan& p0 = *plans[0];
+    std::vector<uint64_t> key;
+    for (const HostPlan* q : plans) key.push_back(q->id);
+    if (!(p0.rough_launch && p0.rough_launch->key == key && p0.rough_launch->backend == static_cast<int>(backend)))
+        prepare_rough(plans, backend);
+    RoughLaunch& L = *p0.rough_launch;
+    const unsigned long long none = ~0ull;
+    check(hipMemcpyAsync(L.d_err.ptr, &none, sizeof(none), hipMemcpyHostToDevice, g_stream), "error word");
+    const float ms = timed([&] {
+        check(backend == PE_BACKEND_GPU_FP64
+                  ? launch_rough_f64(L.a, L.pa, L.grid, static_cast<uint32_t>(L.items),
+                                     static_cast<unsigned long long*>(L.d_err.ptr), g_stream)
+                  : launch_rough_f32(L.a, L.pa, L.grid, static_cast<uint32_t>(L.items),
+                                     static_cast<unsigned long long*>(L.d_err.ptr), g_stream),
+              "rough kernel launch");
+    });
+    check(hipMemcpy(L.host.data(), L.d_partials.ptr, L.host.size() * sizeof(BlockStat), hipMemcpyDeviceToHost),
+          "partials");
+    unsigned long long err = none;
+    check(hipMemcpy(&err, L.d_err.ptr, sizeof(err), hipMemcpyDeviceToHost), "error word");
+    if (err != none)
+        throw PlanError(PE_ERR_NONFINITE_STATE, "sampling unit " + std::to_string(err) + ": non-finite lifted Heston path");
+    const size_t per = static_cast<size_t>(L.scrambles) * L.chunks;
+    for (size_t k = 0; k < plans.size(); ++k) {
+        std::vector<BlockStat> mine(L.host.begin() + static_cast<long>(k * per),
+                                    L.host.begin() + static_cast<long>((k + 1) * per));
+        pe_result* out = outs + k;
+        std::memset(out, 0, sizeof(*out));
+        assemble(*plans[k], mine, L.chunks, L.scrambles, 1, 0, 1, out);
+        out->time_steps = L.n;
+        out->path_steps = out->evaluated_paths * L.n;
+        out->sobol_dimension = L.rqmc ? 2u * L.n : 0u;
+        out->device_ms = ms;
+    }
+    g_kernel_ms = ms;
+    g_kernel_launches = 2;
+}
+
+void prepare_rough(std::vector<HostPlan*>& plans, pe_backend backend) {
+    HostPlan& p0 = *plans[0];
     const pe_engine& e = p0.req.engine;
     std::vector<double> events;
     for (const HostPlan* q : plans) {
@@ -410,7 +470,13 @@ void evaluate_rough(std::vector<HostPlan*>& plans, pe_backend backend, pe_result
     const int passes = e.antithetic ? 2 : 1;
     const uint64_t total_paths = e.units * scrambles * passes;
     const bool fp64 = backend == PE_BACKEND_GPU_FP64;
-    DeviceBuffer d_steps, d_den, d_rden, d_store_nodes, d_store, d_sobol, d_err, d_products, d_items, d_partials;
+    std::unique_ptr<RoughLaunch, RoughLaunchDeleter> L(new RoughLaunch());
+    L->key.clear();
+    for (const HostPlan* q : plans) L->key.push_back(q->id);
+    L->backend = static_cast<int>(backend);
+    DeviceBuffer &d_steps = L->d_steps, &d_den = L->d_den, &d_rden = L->d_rden, &d_store_nodes = L->d_store_nodes,
+                 &d_store = L->d_store, &d_sobol = L->d_sobol, &d_err = L->d_err, &d_products = L->d_products,
+                 &d_items = L->d_items, &d_partials = L->d_partials;
     d_steps.upload(steps);
     d_den.upload(den);
     d_rden.upload(rden);
@@ -484,36 +550,18 @@ void evaluate_rough(std::vector<HostPlan*>& plans, pe_backend backend, pe_result
     pa.total_paths = total_paths;
     pa.antithetic = a.antithetic;
     d_err.reserve(sizeof(unsigned long long));
-    const unsigned long long none = ~0ull;
-    check(hipMemcpyAsync(d_err.ptr, &none, sizeof(none), hipMemcpyHostToDevice, g_stream), "error word");
     const uint64_t blocks = (e.units + kBlockThreads - 1) / kBlockThreads;
-    const dim3 grid(static_cast<uint32_t>(std::min<uint64_t>(blocks, 1u << 14)), scrambles, 1u);
-    const float ms = timed([&] {
-        check(fp64 ? launch_rough_f64(a, pa, grid, static_cast<uint32_t>(items.size()),
-                                      static_cast<unsigned long long*>(d_err.ptr), g_stream)
-                   : launch_rough_f32(a, pa, grid, static_cast<uint32_t>(items.size()),
-                                      static_cast<unsigned long long*>(d_err.ptr), g_stream),
-              "rough kernel launch");
-    });
-    std::vector<BlockStat> host(items.size());
-    check(hipMemcpy(host.data(), d_partials.ptr, host.size() * sizeof(BlockStat), hipMemcpyDeviceToHost), "partials");
-    unsigned long long err = none;
-    check(hipMemcpy(&err, d_err.ptr, sizeof(err), hipMemcpyDeviceToHost), "error word");
-    if (err != none)
-        throw PlanError(PE_ERR_NONFINITE_STATE, "sampling unit " + std::to_string(err) + ": non-finite lifted Heston path");
-    for (size_t k = 0; k < plans.size(); ++k) {
-        std::vector<BlockStat> mine(host.begin() + static_cast<long>(k * scrambles * chunks),
-                                    host.begin() + static_cast<long>((k + 1) * scrambles * chunks));
-        pe_result* out = outs + k;
-        std::memset(out, 0, sizeof(*out));
-        assemble(*plans[k], mine, chunks, scrambles, 1, 0, 1, out);
-        out->time_steps = n;
-        out->path_steps = out->evaluated_paths * n;
-        out->sobol_dimension = rqmc ? 2u * n : 0u;
-        out->device_ms = ms;
-    }
-    g_kernel_ms = ms;
-    g_kernel_launches = 2;
+    L->grid = dim3(static_cast<uint32_t>(std::min<uint64_t>(blocks, 1u << 14)), scrambles, 1u);
+    L->a = a;
+    L->pa = pa;
+    L->n = n;
+    L->chunks = chunks;
+    L->scrambles = scrambles;
+    L->items = items.size();
+    L->rqmc = rqmc;
+    L->host.resize(items.size());
+    (void)fp64;
+    p0.rough_launch.reset(L.release());
 }
 
 void upload_lv_tables(HostPlan& p) {

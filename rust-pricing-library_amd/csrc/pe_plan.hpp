@@ -116,11 +116,16 @@ struct BsLaunch;
 struct BsLaunchDeleter {
     void operator()(BsLaunch* l) const;
 };
+struct RoughLaunch;
+struct RoughLaunchDeleter {
+    void operator()(RoughLaunch* l) const;
+};
 
 struct HostPlan {
     uint64_t id = 0;                 // unique per compiled plan, never reused
     std::unique_ptr<LvLaunch> lv_launch;
     std::unique_ptr<BsLaunch, BsLaunchDeleter> bs_launch;
+    std::unique_ptr<RoughLaunch, RoughLaunchDeleter> rough_launch;
     pe_request req{};
     std::vector<int32_t> dates;
     std::vector<double> weights, fixings;

@@ -110,3 +110,31 @@ def test_bridge_changes_paths_not_the_law():
     assert vals[0].value != vals[1].value
     for e in vals:
         assert abs(e.value - closed) <= 4.0 * e.standard_error + 1e-6
+
+
+def test_scheme_amplifies_last_bit_perturbations():
+    """The truncated lifted Heston scheme is chaotic at the smile parameters
+    (2 kappa theta < nu^2: the variance is truncated at zero often).  A
+    relative perturbation of 1e-12 in the spot shocks moves terminal forwards
+    of some paths by orders of magnitude more, so precisions (or libm ulps)
+    can agree only in distribution, not path by path."""
+    import ctypes as C
+
+    w, x = from_rough(0.1, 20, 2.5)
+    model = abi.lifted_heston(0.02, 0.3, 0.02, 0.3, -0.7, w, x)
+    n = 256
+    times = [2.0 * i / n for i in range(n + 1)]
+    rng = np.random.default_rng(7)
+    worst = 0.0
+    for _ in range(64):
+        z = rng.standard_normal(2 * n)
+        zp = z.copy()
+        zp[:n] *= 1.0 + 1e-12
+        outs = []
+        for zz in (z, zp):
+            f = (C.c_double * (n + 1))()
+            v = (C.c_double * (n + 1))()
+            assert oracle().oracle_lifted_path(C.byref(model), darr(times), n + 1, 100.0, darr(list(zz)), f, v) == 0
+            outs.append(f[n])
+        worst = max(worst, abs(outs[0] - outs[1]) / outs[0])
+    assert worst > 1e-9  # amplified beyond 1000x the input perturbation
