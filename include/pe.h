@@ -109,7 +109,8 @@ typedef enum pe_product_kind {
     PE_PRODUCT_DIGITAL = 1,
     PE_PRODUCT_BARRIER = 2,
     PE_PRODUCT_ASIAN = 3,
-    PE_PRODUCT_LOOKBACK = 4
+    PE_PRODUCT_LOOKBACK = 4,
+    PE_PRODUCT_AMERICAN = 5     /* AmericanVanillaSpec: exercise dates in dates[], the last at expiry */
 } pe_product_kind;
 
 typedef enum pe_side { PE_CALL = 0, PE_PUT = 1 } pe_side;
@@ -173,12 +174,41 @@ typedef struct pe_risk {
     double smoothing_half_width;
 } pe_risk;
 
+/* Continuation-value regression of an early-exercise product. */
+typedef enum pe_lsm_method {
+    PE_LSM_POLY_CPQR = 0,           /* the reference's column-pivoted Householder QR on
+                                       standardized monomials (crates/pricing/src/engine/mc/lsm.rs:1928-2568) */
+    PE_LSM_POLY_LAGUERRE = 1,       /* weighted Laguerre basis, Gram + FP64 Cholesky (no reference counterpart) */
+    PE_LSM_MLP = 2                  /* MLP continuation network, BF16 weights (no reference counterpart) */
+} pe_lsm_method;
+
+/* LsmConfig (crates/pricing/src/engine/mc/lsm.rs, api request "lsm"): the
+ * training engine (domain LsmTrain for pseudo-MC), the spot-feature basis
+ * PolynomialBasisSpec(1, max_degree), CPQR rank tolerances, the ITM
+ * tolerance and the regression matrix element limit; plus the device
+ * method and the MLP hyper-parameters. */
+typedef struct pe_lsm {
+    pe_engine training_engine;
+    uint32_t max_degree;
+    double abs_rank_tolerance;
+    double rel_rank_tolerance;
+    double itm_abs_tolerance;
+    uint64_t max_matrix_elements;
+    pe_lsm_method method;
+    uint32_t mlp_hidden;            /* hidden width (64) */
+    uint32_t mlp_epochs;            /* Adam epochs per exercise date (4) */
+    uint32_t mlp_batch;             /* mini-batch rows (8192) */
+    double mlp_learning_rate;       /* Adam step (1e-3) */
+    uint64_t mlp_seed;              /* initialisation and shuffling */
+} pe_lsm;
+
 typedef struct pe_request {
     pe_market market;
     pe_model model;
     pe_product product;
     pe_engine engine;
     pe_risk risk;
+    pe_lsm lsm;                     /* early-exercise products only */
 } pe_request;
 
 /* ExecutionPolicy (crates/pricing/src/engine/mc/execution.rs:20-68). */

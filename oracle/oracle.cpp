@@ -18,6 +18,8 @@
 // LV valuation:     crates/pricing/src/engine/processes/local_vol_valuation.rs:161-247
 // Estimates:        crates/pricing/src/engine/risk/report.rs:454-473
 // Rough (lifted Heston): crates/pricing/src/engine/risk/rough_volatility.rs:84-342
+// Early exercise (LSM): crates/pricing/src/engine/risk/early_exercise.rs:24-300,
+//                   crates/pricing/src/engine/mc/early_exercise.rs:28-176, 345-388, 616-712
 //
 // Parity status per function is recorded in tests/test_oracle_*.py.
 #include <algorithm>
@@ -34,6 +36,7 @@
 
 #include "oracle_barrier.hpp"
 #include "oracle_dividend.hpp"
+#include "oracle_lsm.hpp"
 #include "oracle_market.hpp"
 #include "oracle_numerics.hpp"
 #include "oracle_payoff.hpp"
@@ -85,6 +88,7 @@ struct Plan {
     Smooth smooth{1.0};
     bool continuous = false;
     ContinuousBarrier cb;
+    bool american = false;
 
     // local vol
     bool local_vol = false;
@@ -193,8 +197,10 @@ static void compile_plan(Plan& p, const pe_request& r, const pe_exec_policy* pol
     const pe_risk& rk = r.risk;
     p.smoothing = rk.smoothing != 0;
     p.smooth.h = rk.smoothing_half_width;
-    // A continuous barrier is priced by its bridge runtime, not by the graph.
-    if (!p.continuous) p.tape = build_tape(pr, idx, p.smoothing, rk.smoothing_half_width, jump_obs);
+    // A continuous barrier is priced by its bridge runtime and an American by
+    // its exercise policy, not by the single-output graph.
+    p.american = pr.kind == PE_PRODUCT_AMERICAN;
+    if (!p.continuous && !p.american) p.tape = build_tape(pr, idx, p.smoothing, rk.smoothing_half_width, jump_obs);
 
     // Affine coordinates per observation (compile/simulation.rs:308-338).
     p.obs_coord.clear();
@@ -767,7 +773,11 @@ static uint32_t simulation_dimension(const Plan& p) {
     return p.local_vol ? static_cast<uint32_t>(p.time_grid.steps()) : static_cast<uint32_t>(p.obs_times.size());
 }
 
+struct LsmOutcome;
+static void evaluate_american(const Plan& p, pe_result* out, LsmOutcome* keep);
+
 static void evaluate_plan(const Plan& p, pe_result* out) {
+    if (p.american) return evaluate_american(p, out, nullptr);
     std::memset(out, 0, sizeof(*out));
     const pe_engine& e = p.req.engine;
     std::unique_ptr<BridgePlan> bridge;
@@ -980,6 +990,150 @@ static void evaluate_rough(const RoughPlan& rp, pe_result* outs) {
         out->path_steps = paths * n;
         out->sobol_dimension = e.kind == PE_ENGINE_RQMC ? dim : 0;
     }
+}
+
+// ----------------------------------------------------- early exercise --
+struct LsmMatrices {  // LsmPathMatrices: date-major immediate values, spot features
+    size_t paths = 0;
+    std::vector<double> immediate, features;
+};
+
+// write_lsm_trajectory (mc/early_exercise.rs:345-388) for a BS/Black-76 plan.
+static void write_trajectory(const Plan& p, const std::vector<double>& z, size_t path, LsmMatrices& m) {
+    bs_observations(p, z.data(), p.spot, p.volatility, tl_obs);
+    const pe_product& pr = p.req.product;
+    const size_t dates = p.obs_times.size();
+    for (size_t d = 0; d < dates; ++d) {
+        const double s = tl_obs[d].post;
+        const double signed_intrinsic = pr.side == PE_CALL ? s - pr.strike : pr.strike - s;
+        m.immediate[d * m.paths + path] = (signed_intrinsic >= 0.0 ? signed_intrinsic : 0.0) * pr.notional;
+        if (d + 1 == dates) continue;
+        m.features[d * m.paths + path] = s;
+    }
+}
+
+static LsmMatrices lsm_matrices(const Plan& p, const pe_engine& e, uint32_t domain) {
+    LsmMatrices m;
+    const size_t mult = e.antithetic ? 2 : 1;
+    const size_t dates = p.obs_times.size();
+    const uint32_t dim = static_cast<uint32_t>(dates);
+    std::vector<double> z(dim), mate(dim);
+    auto both = [&](size_t base) {
+        write_trajectory(p, z, base, m);
+        if (mult == 2) {
+            for (size_t i = 0; i < z.size(); ++i) mate[i] = -z[i];
+            write_trajectory(p, mate, base + 1, m);
+        }
+    };
+    if (e.kind == PE_ENGINE_PSEUDO) {
+        m.paths = e.units * mult;
+        m.immediate.assign(dates * m.paths, 0.0);
+        m.features.assign((dates - 1) * m.paths, 0.0);
+        for (uint64_t u = 0; u < e.units; ++u) {
+            for (uint32_t d = 0; d < dim; ++d)
+                z[d] = p.total_variance == 0.0 ? 0.0 : philox_normal(e.seed, u, d, static_cast<Domain>(domain));
+            both(u * mult);
+        }
+    } else {
+        RqmcPlan q;
+        q.compile(g_directions, e.units, e.scrambles, e.seed, dim);
+        m.paths = static_cast<size_t>(e.scrambles) * e.units * mult;
+        m.immediate.assign(dates * m.paths, 0.0);
+        m.features.assign((dates - 1) * m.paths, 0.0);
+        for (uint32_t s = 0; s < e.scrambles; ++s)
+            for (uint64_t u = 0; u < e.units; ++u) {
+                for (uint32_t d = 0; d < dim; ++d)
+                    z[d] = p.total_variance == 0.0 ? 0.0 : inverse_standard_normal(q.uniform(s, u, d));
+                both((static_cast<size_t>(s) * e.units + u) * mult);
+            }
+    }
+    return m;
+}
+
+struct LsmOutcome {
+    TrainedPolicy trained;
+    ValuedPolicy valued;
+    std::vector<double> unit_values;
+    double in_sample_value = 0.0;
+};
+
+static std::vector<double> collapse_antithetic(const std::vector<double>& v, bool anti) {
+    if (!anti) return v;
+    std::vector<double> out;
+    for (size_t i = 0; i + 1 < v.size(); i += 2) out.push_back((v[i] + v[i + 1]) * 0.5);
+    return out;
+}
+
+static std::vector<double> rqmc_replicates(const std::vector<double>& v, const pe_engine& e) {
+    const size_t mult = e.antithetic ? 2 : 1;
+    const size_t per = e.units * mult;
+    std::vector<double> out;
+    for (uint32_t s = 0; s < e.scrambles; ++s) {
+        NeumaierSum sum;
+        const double* x = &v[s * per];
+        if (mult == 2)
+            for (size_t i = 0; i < per; i += 2) sum.add((x[i] + x[i + 1]) * 0.5);
+        else
+            for (size_t i = 0; i < per; ++i) sum.add(x[i]);
+        out.push_back(sum.total() / static_cast<double>(e.units));
+    }
+    return out;
+}
+
+// execute_early_exercise / execute_early_exercise_rqmc (risk/early_exercise.rs:24-300).
+static void evaluate_american(const Plan& p, pe_result* out, LsmOutcome* keep) {
+    const pe_lsm& l = p.req.lsm;
+    const pe_engine& ve = p.req.engine;
+    const pe_engine& te = l.training_engine;
+    if (te.kind != ve.kind) throw std::invalid_argument("training and valuation engines must be of one kind");
+    if (l.method != PE_LSM_POLY_CPQR) throw std::invalid_argument("the oracle restates the CPQR policy only");
+    LsmSettings s;
+    s.basis.init(1, l.max_degree);
+    s.abs_tol = l.abs_rank_tolerance;
+    s.rel_tol = l.rel_rank_tolerance;
+    s.itm_tol = l.itm_abs_tolerance;
+    s.max_elements = l.max_matrix_elements ? l.max_matrix_elements : UINT64_MAX;
+    const size_t dates = p.obs_times.size();
+    std::vector<double> discount;
+    for (uint32_t i = 0; i < p.req.product.date_count; ++i)
+        discount.push_back(p.market.discount_curve.discount(static_cast<double>(p.req.product.dates[i]) / 365.0));
+    if (discount.size() != dates) throw std::invalid_argument("exercise dates must be future and distinct");
+    LsmOutcome o;
+    {
+        const LsmMatrices train = lsm_matrices(p, te, kLsmTrain);
+        o.trained = train_exercise_policy(s, dates, train.paths, train.immediate, train.features, discount);
+    }
+    const LsmMatrices val = lsm_matrices(p, ve, kValuation);
+    o.valued = value_exercise_policy(o.trained.decisions, dates, val.paths, 1, val.immediate, val.features, discount);
+    std::vector<double> in_sample(o.trained.realized_cashflows.size());
+    for (size_t i = 0; i < in_sample.size(); ++i)
+        in_sample[i] = o.trained.realized_cashflows[i] * discount[o.trained.stopping_indices[i]];
+    uint64_t independent;
+    if (ve.kind == PE_ENGINE_PSEUDO) {
+        o.unit_values = collapse_antithetic(o.valued.discounted_cashflows, ve.antithetic != 0);
+        independent = ve.units;
+        const std::vector<double> ins = collapse_antithetic(in_sample, te.antithetic != 0);
+        o.in_sample_value = Statistics::two_pass(ins).sum.total() / static_cast<double>(ins.size());
+    } else {
+        o.unit_values = rqmc_replicates(o.valued.discounted_cashflows, ve);
+        independent = ve.scrambles;
+        const std::vector<double> ins = rqmc_replicates(in_sample, te);
+        o.in_sample_value = Statistics::two_pass(ins).sum.total() / static_cast<double>(ins.size());
+    }
+    const Statistics st = Statistics::two_pass(o.unit_values);
+    std::memset(out, 0, sizeof(*out));
+    out->estimates[PE_C_PRICE] = estimate(st, independent, 1.0);
+    out->component_sum[PE_C_PRICE] = st.sum.total();
+    out->component_mean[PE_C_PRICE] = st.moments.mean;
+    out->component_m2[PE_C_PRICE] = st.moments.m2;
+    const double sv = st.moments.has_variance() ? st.moments.sample_variance() : 0.0;
+    out->sampling_variance = sv;
+    out->estimator_variance = sv / static_cast<double>(independent);
+    out->independent_units = independent;
+    out->evaluated_paths = val.paths;
+    out->time_steps = static_cast<uint32_t>(dates);
+    out->path_steps = val.paths * dates;
+    if (keep) *keep = std::move(o);
 }
 
 }  // namespace oracle
@@ -1397,6 +1551,84 @@ int oracle_rough_evaluate(const pe_request* const* reqs, uint32_t count, const p
     } catch (const std::exception& e) {
         return fail(e, PE_ERR_UNSUPPORTED);
     }
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// The American plan's exercise policy: per non-terminal date, `row` of
+// `diag` holds (itm rows, rank, rank threshold, residual sum of squares,
+// feature mean, population variance, coefficients[0..3], diagonals[0..3],
+// pivot order[0..3], continue-all flag) = 20 values; returns the date count.
+int oracle_lsm_evaluate(const pe_request* req, const pe_exec_policy* policy, pe_result* out, double* diag,
+                        uint32_t capacity, double* in_sample, uint64_t* exercise_counts) {
+    try {
+        Plan p;
+        compile_plan(p, *req, policy);
+        if (!p.american) throw std::invalid_argument("not an early-exercise product");
+        LsmOutcome o;
+        evaluate_american(p, out, &o);
+        if (in_sample) *in_sample = o.in_sample_value;
+        if (exercise_counts)
+            for (size_t d = 0; d < o.valued.exercise_counts.size(); ++d) exercise_counts[d] = o.valued.exercise_counts[d];
+        const size_t n = o.trained.decisions.size();
+        for (size_t d = 0; d < n && d < capacity && diag; ++d) {
+            double* r = diag + 20 * d;
+            const Decision& dec = o.trained.decisions[d];
+            for (int k = 0; k < 20; ++k) r[k] = 0.0;
+            r[0] = static_cast<double>(dec.itm_rows);
+            r[19] = dec.continue_all ? 1.0 : 0.0;
+            if (dec.continue_all) continue;
+            const Regression& m = dec.model;
+            r[1] = static_cast<double>(m.rank);
+            r[2] = m.rank_threshold;
+            r[3] = m.residual_sum_squares;
+            r[4] = m.scalings[0].mean;
+            r[5] = m.scalings[0].population_variance;
+            for (size_t k = 0; k < 4 && k < m.coefficients.size(); ++k) r[6 + k] = m.coefficients[k];
+            for (size_t k = 0; k < 4 && k < m.diagonal_abs.size(); ++k) r[10 + k] = m.diagonal_abs[k];
+            for (size_t k = 0; k < 4 && k < m.pivot_order.size(); ++k) r[14 + k] = static_cast<double>(m.pivot_order[k]);
+        }
+        return static_cast<int>(n);
+    } catch (const std::exception& e) { return -fail(e, 1); }
+}
+
+// fit_cpqr on a row-major matrix: out = rank, rss, coefficients[cols], pivots[cols].
+int oracle_cpqr(const double* matrix, uint32_t rows, uint32_t cols, const double* target, double abs_tol,
+                double rel_tol, double* out) {
+    try {
+        const Cpqr f = fit_cpqr(std::vector<double>(matrix, matrix + static_cast<size_t>(rows) * cols), rows, cols,
+                                std::vector<double>(target, target + rows), abs_tol, rel_tol);
+        out[0] = static_cast<double>(f.rank);
+        out[1] = f.residual_sum_squares;
+        for (uint32_t c = 0; c < cols; ++c) {
+            out[2 + c] = f.coefficients[c];
+            out[2 + cols + c] = static_cast<double>(f.pivot_order[c]);
+        }
+        return 0;
+    } catch (const std::exception& e) { return fail(e, 1); }
+}
+
+// PolynomialBasisSpec exponents (row-major, feature_count per column); returns the column count.
+int oracle_basis_exponents(uint32_t features, uint32_t degree, uint32_t* out, uint32_t capacity) {
+    Basis b;
+    b.init(features, degree);
+    size_t k = 0;
+    for (const auto& e : b.exponents)
+        for (uint32_t v : e)
+            if (k < capacity) out[k++] = v;
+    return static_cast<int>(b.exponents.size());
+}
+
+// FeatureScaling::fit: out = mean, population variance, scale, zero-scale threshold, inactive.
+void oracle_feature_scaling(const double* values, uint32_t n, double* out) {
+    const Scaling s = Scaling::fit(std::vector<double>(values, values + n));
+    out[0] = s.mean;
+    out[1] = s.population_variance;
+    out[2] = s.scale;
+    out[3] = s.zero_scale_threshold;
+    out[4] = s.inactive ? 1.0 : 0.0;
 }
 
 }  // extern "C"

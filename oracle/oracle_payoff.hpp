@@ -312,6 +312,7 @@ inline std::vector<int32_t> observation_days(const pe_product& p) {
                 for (uint32_t i = 0; i < p.date_count; ++i) add(p.dates[i]);
             break;
         case PE_PRODUCT_LOOKBACK:
+        case PE_PRODUCT_AMERICAN:
             for (uint32_t i = 0; i < p.date_count; ++i) add(p.dates[i]);
             break;
         default:

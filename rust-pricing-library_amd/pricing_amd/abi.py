@@ -28,6 +28,7 @@ PE_PRODUCT_DIGITAL = 1
 PE_PRODUCT_BARRIER = 2
 PE_PRODUCT_ASIAN = 3
 PE_PRODUCT_LOOKBACK = 4
+PE_PRODUCT_AMERICAN = 5
 
 PE_CALL = 0
 PE_PUT = 1
@@ -134,6 +135,28 @@ class pe_risk(C.Structure):
     ]
 
 
+PE_LSM_POLY_CPQR = 0
+PE_LSM_POLY_LAGUERRE = 1
+PE_LSM_MLP = 2
+
+
+class pe_lsm(C.Structure):
+    _fields_ = [
+        ("training_engine", pe_engine),
+        ("max_degree", C.c_uint32),
+        ("abs_rank_tolerance", C.c_double),
+        ("rel_rank_tolerance", C.c_double),
+        ("itm_abs_tolerance", C.c_double),
+        ("max_matrix_elements", C.c_uint64),
+        ("method", C.c_int),
+        ("mlp_hidden", C.c_uint32),
+        ("mlp_epochs", C.c_uint32),
+        ("mlp_batch", C.c_uint32),
+        ("mlp_learning_rate", C.c_double),
+        ("mlp_seed", C.c_uint64),
+    ]
+
+
 class pe_request(C.Structure):
     _fields_ = [
         ("market", pe_market),
@@ -141,6 +164,7 @@ class pe_request(C.Structure):
         ("product", pe_product),
         ("engine", pe_engine),
         ("risk", pe_risk),
+        ("lsm", pe_lsm),
     ]
 
 

@@ -142,6 +142,32 @@ class Request:
         e.brownian_bridge = int(bool(brownian_bridge))
         return self
 
+    def set_lsm(self, training, antithetic=False, brownian_bridge=False, max_degree=3, abs_rank_tolerance=1e-14,
+                rel_rank_tolerance=1e-12, itm_abs_tolerance=0.0, max_matrix_elements=1 << 40,
+                method=abi.PE_LSM_POLY_CPQR, mlp_hidden=64, mlp_epochs=4, mlp_batch=8192, mlp_learning_rate=1e-3,
+                mlp_seed=0x4D4C50):
+        """LsmConfig: training = (engine kind, seed, units or points, scrambles)."""
+        l = self.c.lsm
+        kind, seed, units, scrambles = training
+        l.training_engine.kind = kind
+        l.training_engine.seed = int(seed)
+        l.training_engine.units = int(units)
+        l.training_engine.scrambles = int(scrambles)
+        l.training_engine.antithetic = int(bool(antithetic))
+        l.training_engine.brownian_bridge = int(bool(brownian_bridge))
+        l.max_degree = int(max_degree)
+        l.abs_rank_tolerance = float(abs_rank_tolerance)
+        l.rel_rank_tolerance = float(rel_rank_tolerance)
+        l.itm_abs_tolerance = float(itm_abs_tolerance)
+        l.max_matrix_elements = int(max_matrix_elements)
+        l.method = int(method)
+        l.mlp_hidden = int(mlp_hidden)
+        l.mlp_epochs = int(mlp_epochs)
+        l.mlp_batch = int(mlp_batch)
+        l.mlp_learning_rate = float(mlp_learning_rate)
+        l.mlp_seed = int(mlp_seed)
+        return self
+
     def set_risk(self, delta=False, vega=False, gamma_bump=None, gamma_relative=True,
                  smoothing_half_width=None):
         r = self.c.risk
@@ -217,6 +243,10 @@ class Request:
             self.set_product(abi.PE_PRODUCT_ASIAN, side, (last - valuation).days, pr["strike"], notional,
                              _days(valuation, pr.get("payment_date", last.isoformat())), dates=dates,
                              weights=weights, fixings=fixings)
+        elif kind == "american_vanilla":
+            dates = [_days(valuation, d) for d in pr["exercise_dates"]]
+            self.set_product(abi.PE_PRODUCT_AMERICAN, side, _days(valuation, pr["expiry"]), pr["strike"], notional,
+                             _days(valuation, pr.get("payment_date", pr["expiry"])), dates=dates)
         elif kind == "fixed_lookback":
             dates = [_days(valuation, d) for d in pr["monitoring_dates"]]
             self.set_product(abi.PE_PRODUCT_LOOKBACK, side, max(dates), pr["strike"], notional,
@@ -234,6 +264,19 @@ class Request:
                           vr.get("antithetic", False), vr.get("brownian_bridge", False))
         else:
             raise UnsupportedRequest(f"engine {eng['type']}")
+        lsm = doc.get("lsm")
+        if lsm:
+            t = lsm["training_engine"]
+            tvr = t.get("variance_reduction", {})
+            if t["type"] == "pseudo_monte_carlo":
+                training = (abi.PE_ENGINE_PSEUDO, t["master_seed"], t["independent_sampling_units"], 0)
+            else:
+                training = (abi.PE_ENGINE_RQMC, t["master_scramble_seed"], t["points_per_scramble"], t["scramble_count"])
+            if [v["type"] for v in lsm["state_variables"]] != ["spot"] or lsm["basis"]["feature_count"] != 1:
+                raise UnsupportedRequest("LSM state variables other than the spot")
+            self.set_lsm(training, tvr.get("antithetic", False), tvr.get("brownian_bridge", False),
+                         lsm["basis"]["max_degree"], lsm["cpqr"]["abs_rank_tolerance"],
+                         lsm["cpqr"]["rel_rank_tolerance"], lsm["itm_abs_tolerance"], lsm["max_matrix_elements"])
         risk = doc.get("risk", {})
         gamma = risk.get("gamma")
         gamma_bump = None

@@ -67,6 +67,11 @@ def oracle():
     lib.oracle_jump_hit.argtypes = [C.c_int] + [C.c_double] * 4 + [D]
     lib.oracle_dividend_event.argtypes = [C.c_double] * 6 + [D]
     lib.oracle_plan_coordinates.argtypes = [C.POINTER(abi.pe_request), D, C.c_uint32, D]
+    lib.oracle_lsm_evaluate.argtypes = [C.POINTER(abi.pe_request), C.POINTER(abi.pe_exec_policy),
+                                        C.POINTER(abi.pe_result), D, C.c_uint32, D, C.POINTER(C.c_uint64)]
+    lib.oracle_cpqr.argtypes = [D, C.c_uint32, C.c_uint32, D, C.c_double, C.c_double, D]
+    lib.oracle_basis_exponents.argtypes = [C.c_uint32, C.c_uint32, U, C.c_uint32]
+    lib.oracle_feature_scaling.argtypes = [D, C.c_uint32, D]
     lib.oracle_lifted_from_rough.argtypes = [C.c_double, C.c_uint32, C.c_double, D, D]
     lib.oracle_lifted_path.argtypes = [C.POINTER(abi.pe_lifted_heston), D, C.c_uint32, C.c_double, D, D, D]
     lib.oracle_rough_evaluate.argtypes = [C.POINTER(C.POINTER(abi.pe_request)), C.c_uint32,

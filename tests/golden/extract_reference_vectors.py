@@ -10,6 +10,8 @@ from
   fixtures/path-dependence/reference-cases-v0.1.json  (smoothing, extrema, bridge, smoothed bridge, jumps)
   fixtures/rough-volatility/lifted-factor-prices.json  (finite lift kernels)
   fixtures/rough-volatility/two-step-prices.json  (lifted Heston two-step call prices)
+  fixtures/replay/early_exercise-windows-x86_64.json  (LSM requests, prices, fitted decision models)
+  fixtures/early-exercise/reference-cases-v0.1.json  (decision, basis, scaling, QR cases)
 
 The tests read the JSON this script writes; nothing at test time reads the
 reference tree.
@@ -72,6 +74,18 @@ def main():
     out["two_step_lifted_heston"] = {
         "spot": two["spot"], "time_nodes": two["time_nodes"], "strikes": two["strikes"],
         "case": next(c for c in two["cases"] if c["name"] == "lifted_heston")}
+    with open(os.path.join(ROOT, "fixtures", "replay", "early_exercise-windows-x86_64.json")) as f:
+        ee = json.load(f)
+    out["early_exercise_replay"] = [{
+        "name": c["name"],
+        "request": c["request"],
+        "reduction_block_size": c["plan"]["reduction_block_size"],
+        "value": c["result"]["value"],
+        "early_exercise": {k: v for k, v in c["result"]["monte_carlo"]["early_exercise"].items()
+                           if k in ("decision_models", "exercise_counts", "in_sample_value", "stopping_indices")},
+    } for c in ee["cases"]]
+    with open(os.path.join(ROOT, "fixtures", "early-exercise", "reference-cases-v0.1.json")) as f:
+        out["early_exercise_cases"] = json.load(f)
     with open(os.path.join(HERE, "reference_vectors.json"), "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
         f.write("\n")
