@@ -295,6 +295,55 @@ pe_status pe_plans_evaluate_shared(const pe_plan* const* plans, uint32_t count,
 pe_status pe_batch_evaluate(const pe_plan* const* plans, uint32_t count, pe_backend backend,
                             pe_result* results);
 
+/* --------------------------------------------------------- early exercise -- */
+
+/* One exercise date's fitted continuation model (ExerciseDecisionModel,
+ * crates/pricing/src/engine/mc/lsm.rs:776-819 and its regression diagnostics,
+ * :2067-2220).  coefficients are per basis column in the original order
+ * (exponent k of the standardized spot, or weighted Laguerre function k);
+ * diagonal_abs are |R_pp| in pivot order. */
+typedef struct pe_lsm_decision {
+    int32_t continue_all;           /* no in-the-money training row */
+    uint32_t rank;
+    uint64_t itm_rows;
+    double coefficients[4];
+    double diagonal_abs[4];
+    uint32_t pivot_order[4];
+    double feature_mean;            /* FeatureScaling of the spot (monomial basis) */
+    double feature_population_variance;
+    double feature_scale;
+    int32_t feature_inactive;
+    int32_t pad;
+    double rank_threshold;
+    double residual_sum_squares;
+} pe_lsm_decision;
+
+/* Early-exercise diagnostics of one plan (EarlyExerciseDiagnostics,
+ * crates/pricing/src/engine/risk/early_exercise.rs:24-300): the in-sample
+ * value of the trained policy, valuation exercise counts per date and the
+ * decision model per non-terminal date; plus device time per phase. */
+typedef struct pe_lsm_report {
+    double in_sample_value;
+    double training_ms;             /* training paths + backward induction, HIP events */
+    double valuation_ms;            /* out-of-sample valuation */
+    uint32_t exercise_dates;
+    uint32_t capacity;              /* entries of the caller arrays below */
+    uint64_t* exercise_counts;      /* [capacity] or NULL */
+    pe_lsm_decision* decisions;     /* [capacity] or NULL */
+} pe_lsm_report;
+
+/* Prices `count` American plans by least-squares Monte Carlo
+ * (execute_early_exercise / execute_early_exercise_rqmc,
+ * crates/pricing/src/engine/risk/early_exercise.rs:24-300): training paths
+ * in the LsmTrain domain, backward induction, and an out-of-sample valuation
+ * on independent paths, all on the device.  results[i] equals what
+ * pe_plan_evaluate returns for plans[i]; reports (optional, `count` entries)
+ * receive the policy diagnostics.  The method of each plan's pe_lsm selects
+ * the reference's monomial CPQR policy (solved by normal equations with its
+ * pivot and rank rules), the weighted-Laguerre policy or the MLP policy. */
+pe_status pe_lsm_evaluate(const pe_plan* const* plans, uint32_t count, pe_backend backend, pe_result* results,
+                          pe_lsm_report* reports);
+
 /* ------------------------------------------------- lifted Heston (rough) -- */
 
 /* LiftedHeston (crates/pricing/src/models/rough_volatility.rs:172-212): a

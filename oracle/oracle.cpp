@@ -1086,9 +1086,12 @@ static void evaluate_american(const Plan& p, pe_result* out, LsmOutcome* keep) {
     const pe_engine& ve = p.req.engine;
     const pe_engine& te = l.training_engine;
     if (te.kind != ve.kind) throw std::invalid_argument("training and valuation engines must be of one kind");
-    if (l.method != PE_LSM_POLY_CPQR) throw std::invalid_argument("the oracle restates the CPQR policy only");
+    if (l.method != PE_LSM_POLY_CPQR && l.method != PE_LSM_POLY_LAGUERRE)
+        throw std::invalid_argument("the oracle restates the polynomial policies only");
     LsmSettings s;
     s.basis.init(1, l.max_degree);
+    s.laguerre = l.method == PE_LSM_POLY_LAGUERRE;
+    s.strike = p.req.product.strike;
     s.abs_tol = l.abs_rank_tolerance;
     s.rel_tol = l.rel_rank_tolerance;
     s.itm_tol = l.itm_abs_tolerance;
@@ -1580,6 +1583,16 @@ int oracle_lsm_evaluate(const pe_request* req, const pe_exec_policy* policy, pe_
             r[0] = static_cast<double>(dec.itm_rows);
             r[19] = dec.continue_all ? 1.0 : 0.0;
             if (dec.continue_all) continue;
+            if (dec.laguerre) {
+                const Cpqr& f = dec.lag.fit;
+                r[1] = static_cast<double>(f.rank);
+                r[2] = f.rank_threshold;
+                r[3] = f.residual_sum_squares;
+                for (size_t k = 0; k < 4 && k < f.coefficients.size(); ++k) r[6 + k] = f.coefficients[k];
+                for (size_t k = 0; k < 4 && k < f.diagonal_abs.size(); ++k) r[10 + k] = f.diagonal_abs[k];
+                for (size_t k = 0; k < 4 && k < f.pivot_order.size(); ++k) r[14 + k] = static_cast<double>(f.pivot_order[k]);
+                continue;
+            }
             const Regression& m = dec.model;
             r[1] = static_cast<double>(m.rank);
             r[2] = m.rank_threshold;
@@ -1609,6 +1622,39 @@ int oracle_cpqr(const double* matrix, uint32_t rows, uint32_t cols, const double
         return 0;
     } catch (const std::exception& e) { return fail(e, 1); }
 }
+
+// Normal equations + pivoted Cholesky on a row-major design, same output
+// layout as oracle_cpqr plus the |R_pp| diagonals: out = rank, rss,
+// coefficients[cols], pivots[cols], diagonals[cols].
+int oracle_cholesky_fit(const double* matrix, uint32_t rows, uint32_t cols, const double* target, double abs_tol,
+                        double rel_tol, double* out) {
+    try {
+        const Cpqr f = fit_normal_equations(std::vector<double>(matrix, matrix + static_cast<size_t>(rows) * cols), rows,
+                                            cols, std::vector<double>(target, target + rows), abs_tol, rel_tol);
+        out[0] = static_cast<double>(f.rank);
+        out[1] = f.residual_sum_squares;
+        for (uint32_t c = 0; c < cols; ++c) {
+            out[2 + c] = f.coefficients[c];
+            out[2 + cols + c] = static_cast<double>(f.pivot_order[c]);
+            out[2 + 2 * cols + c] = c < f.diagonal_abs.size() ? f.diagonal_abs[c] : 0.0;
+        }
+        return 0;
+    } catch (const std::exception& e) { return fail(e, 1); }
+}
+
+// fit_cpqr's |R_pp| diagonals (for comparison with the Cholesky fit).
+int oracle_cpqr_diagonals(const double* matrix, uint32_t rows, uint32_t cols, const double* target, double abs_tol,
+                          double rel_tol, double* out) {
+    try {
+        const Cpqr f = fit_cpqr(std::vector<double>(matrix, matrix + static_cast<size_t>(rows) * cols), rows, cols,
+                                std::vector<double>(target, target + rows), abs_tol, rel_tol);
+        for (uint32_t c = 0; c < cols; ++c) out[c] = c < f.diagonal_abs.size() ? f.diagonal_abs[c] : 0.0;
+        return 0;
+    } catch (const std::exception& e) { return fail(e, 1); }
+}
+
+// exp(-x/2) L_k(x), k = 0..degree.
+void oracle_weighted_laguerre(double x, uint32_t degree, double* out) { weighted_laguerre(x, degree, out); }
 
 // PolynomialBasisSpec exponents (row-major, feature_count per column); returns the column count.
 int oracle_basis_exponents(uint32_t features, uint32_t degree, uint32_t* out, uint32_t capacity) {

@@ -207,6 +207,111 @@ inline Cpqr fit_cpqr(const std::vector<double>& matrix, size_t rows, size_t cols
     return fit;
 }
 
+// Least squares through the normal equations with a pivoted Cholesky that
+// follows fit_cpqr's pivot and rank rules: at step p the remaining column of
+// largest residual squared norm (the Schur complement's diagonal; ties to the
+// smaller original index) is chosen, |R_pp| is the square root of that
+// diagonal, the rank keeps the leading |R_pp| above max(abs, rel |R_00|) and
+// the solution uses the first `rank` pivoted columns only.  In exact
+// arithmetic the pivots, |R_pp| and coefficients equal fit_cpqr's; the
+// rounding differs (the Gram squares the condition number).  No reference
+// counterpart: the device solves LSM regressions this way.
+inline Cpqr fit_gram_cholesky(const std::vector<double>& gram, const std::vector<double>& rhs, double yy,
+                              size_t rows, size_t cols, double abs_tol, double rel_tol) {
+    if (rows == 0 || cols == 0) throw std::invalid_argument("empty regression");
+    std::vector<double> s = gram, b = rhs;
+    std::vector<size_t> perm(cols);
+    for (size_t i = 0; i < cols; ++i) perm[i] = i;
+    std::vector<double> l(cols * cols, 0.0);
+    Cpqr fit;
+    const size_t diag_count = std::min(rows, cols);
+    for (size_t p = 0; p < diag_count; ++p) {
+        size_t sel = p;
+        for (size_t c = p + 1; c < cols; ++c) {
+            const double a = s[c * cols + c], m = s[sel * cols + sel];
+            if (a > m || (a == m && perm[c] < perm[sel])) sel = c;
+        }
+        if (sel != p) {
+            for (size_t k = 0; k < cols; ++k) std::swap(s[p * cols + k], s[sel * cols + k]);
+            for (size_t k = 0; k < cols; ++k) std::swap(s[k * cols + p], s[k * cols + sel]);
+            for (size_t k = 0; k < p; ++k) std::swap(l[p * cols + k], l[sel * cols + k]);
+            std::swap(b[p], b[sel]);
+            std::swap(perm[p], perm[sel]);
+        }
+        const double d = s[p * cols + p];
+        if (!(d > 0.0)) {
+            fit.diagonal_abs.push_back(0.0);
+            continue;
+        }
+        const double r = std::sqrt(d);
+        l[p * cols + p] = r;
+        for (size_t i = p + 1; i < cols; ++i) l[i * cols + p] = s[i * cols + p] / r;
+        for (size_t i = p + 1; i < cols; ++i)
+            for (size_t j = p + 1; j < cols; ++j) s[i * cols + j] -= l[i * cols + p] * l[j * cols + p];
+        fit.diagonal_abs.push_back(r);
+    }
+    const double first = fit.diagonal_abs.empty() ? 0.0 : fit.diagonal_abs[0];
+    fit.rank_threshold = std::max(abs_tol, rel_tol * first);
+    size_t rank = 0;
+    while (rank < fit.diagonal_abs.size() && fit.diagonal_abs[rank] > fit.rank_threshold) ++rank;
+    fit.rank = rank;
+    // L_r L_r^T c = b_r: forward then back substitution
+    std::vector<double> z(rank, 0.0), c(cols, 0.0);
+    for (size_t i = 0; i < rank; ++i) {
+        double v = b[i];
+        for (size_t k = 0; k < i; ++k) v -= l[i * cols + k] * z[k];
+        z[i] = v / l[i * cols + i];
+    }
+    for (size_t i = rank; i-- > 0;) {
+        double v = z[i];
+        for (size_t k = i + 1; k < rank; ++k) v -= l[k * cols + i] * c[k];
+        c[i] = v / l[i * cols + i];
+        if (!std::isfinite(c[i])) throw std::runtime_error("Cholesky back substitution");
+    }
+    fit.coefficients.assign(cols, 0.0);
+    double explained = 0.0;
+    for (size_t p = 0; p < rank; ++p) {
+        fit.coefficients[perm[p]] = c[p];
+        explained += c[p] * b[p];
+    }
+    fit.residual_sum_squares = yy - explained;
+    fit.pivot_order = perm;
+    return fit;
+}
+
+// Gram matrix, X^T y and y^T y of a row-major design with Neumaier sums.
+inline Cpqr fit_normal_equations(const std::vector<double>& design, size_t rows, size_t cols,
+                                 const std::vector<double>& target, double abs_tol, double rel_tol) {
+    std::vector<double> gram(cols * cols), rhs(cols);
+    for (size_t i = 0; i < cols; ++i) {
+        for (size_t j = i; j < cols; ++j) {
+            NeumaierSum g;
+            for (size_t r = 0; r < rows; ++r) g.add(design[r * cols + i] * design[r * cols + j]);
+            gram[i * cols + j] = gram[j * cols + i] = g.total();
+        }
+        NeumaierSum t;
+        for (size_t r = 0; r < rows; ++r) t.add(design[r * cols + i] * target[r]);
+        rhs[i] = t.total();
+    }
+    NeumaierSum yy;
+    for (size_t r = 0; r < rows; ++r) yy.add(target[r] * target[r]);
+    return fit_gram_cholesky(gram, rhs, yy.total(), rows, cols, abs_tol, rel_tol);
+}
+
+// Weighted Laguerre functions exp(-x/2) L_k(x), k = 0..degree, by the
+// three-term recurrence (k+1) L_{k+1} = (2k+1-x) L_k - k L_{k-1}.  The
+// Longstaff--Schwartz basis for the American put; x = spot / strike.
+inline void weighted_laguerre(double x, uint32_t degree, double* out) {
+    const double w = std::exp(-0.5 * x);
+    double prev = 0.0, cur = 1.0;
+    for (uint32_t k = 0; k <= degree; ++k) {
+        out[k] = w * cur;
+        const double next = ((2.0 * k + 1.0 - x) * cur - static_cast<double>(k) * prev) / (k + 1.0);
+        prev = cur;
+        cur = next;
+    }
+}
+
 struct Regression {  // PolynomialRegressionModel
     Basis basis;
     std::vector<Scaling> scalings;
@@ -273,15 +378,40 @@ inline Regression fit_polynomial_regression(const Basis& basis, const std::vecto
     return m;
 }
 
+// Weighted-Laguerre continuation model on x = spot / strike (the device's
+// plain method; no reference counterpart).
+struct LaguerreModel {
+    uint32_t degree = 3;
+    double strike = 1.0;
+    Cpqr fit;
+
+    double predict(double spot) const {
+        double phi[16];
+        weighted_laguerre(spot / strike, degree, phi);
+        NeumaierSum p;
+        for (uint32_t k = 0; k <= degree; ++k) p.add(phi[k] * fit.coefficients[k]);
+        return p.total();
+    }
+};
+
+// The reference compares immediate and continuation values on every path, so
+// an out-of-the-money path whose extrapolated continuation is negative stops
+// with a zero payoff.  The Laguerre policy exercises only in the money
+// (immediate > ITM tolerance), as Longstaff and Schwartz do.
 struct Decision {  // ExerciseDecisionModel
     bool continue_all = false;
+    bool laguerre = false;
+    double itm_tol = 0.0;
     Regression model;
+    LaguerreModel lag;
     size_t itm_rows = 0;
 
+    double continuation(const double* f) const { return laguerre ? lag.predict(f[0]) : model.predict(f); }
     bool should_exercise(double immediate, const double* f) const {
         if (!std::isfinite(immediate)) throw std::runtime_error("non-finite decision value");
         if (continue_all) return false;
-        return immediate > model.predict(f);
+        if (laguerre && !(immediate > itm_tol)) return false;
+        return immediate > continuation(f);
     }
 };
 
@@ -289,6 +419,8 @@ struct LsmSettings {
     Basis basis;
     double abs_tol = 1e-14, rel_tol = 1e-12, itm_tol = 0.0;
     uint64_t max_elements = UINT64_MAX;
+    bool laguerre = false;   // weighted Laguerre + Gram/Cholesky instead of monomials + CPQR
+    double strike = 1.0;
 };
 
 inline Decision fit_exercise_decision(const LsmSettings& s, const double* features, size_t rows,
@@ -311,6 +443,18 @@ inline Decision fit_exercise_decision(const LsmSettings& s, const double* featur
         for (size_t i = 0; i < fc; ++i) f[k * fc + i] = features[r * fc + i];
         t[k] = targets[r];
         ++k;
+    }
+    if (s.laguerre) {
+        const uint32_t deg = s.basis.max_degree, cols = deg + 1;
+        if (fc != 1 || cols > 16) throw std::invalid_argument("Laguerre basis: one feature, degree < 16");
+        std::vector<double> design(itm * cols);
+        for (size_t r = 0; r < itm; ++r) weighted_laguerre(f[r] / s.strike, deg, &design[r * cols]);
+        d.laguerre = true;
+        d.itm_tol = s.itm_tol;
+        d.lag.degree = deg;
+        d.lag.strike = s.strike;
+        d.lag.fit = fit_normal_equations(design, itm, cols, t, s.abs_tol, s.rel_tol);
+        return d;
     }
     d.model = fit_polynomial_regression(s.basis, f, itm, t, s.abs_tol, s.rel_tol);
     return d;

@@ -792,6 +792,223 @@ void evaluate_lv(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* o
     g_kernel_launches = 1;
 }
 
+}  // namespace
+
+// ------------------------------------------------------- early exercise --
+// A set of American plans runs as three launches (training paths, one
+// induction workgroup per trade, valuation items); all device state is cached
+// on the set's first plan.
+struct LsmLaunch {
+    std::vector<uint64_t> key;
+    int backend = 0;
+    uint32_t basis = 0;
+    std::vector<LsmTrade> trades;
+    std::vector<LsmItem> train_items, value_items;
+    std::vector<uint32_t> first_partial, chunks, scrambles;
+    DeviceBuffer d_trades, d_dates, d_sobol, d_train_items, d_value_items, store, cash, decisions, counts, in_sample,
+        partials, err;
+    size_t n_decisions = 0, n_counts = 0, n_partials = 0;
+    std::vector<BlockStat> host_partials;
+    std::vector<LsmDecision> host_decisions;
+    std::vector<unsigned long long> host_counts;
+    std::vector<double> host_in_sample;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    ~LsmLaunch() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+void LsmLaunchDeleter::operator()(LsmLaunch* l) const { delete l; }
+
+namespace {
+
+LsmLaunch& prepare_lsm(std::vector<HostPlan*>& plans, pe_backend backend) {
+    HostPlan& p0 = *plans[0];
+    std::vector<uint64_t> key;
+    for (const HostPlan* q : plans) key.push_back(q->id);
+    if (p0.lsm_launch && p0.lsm_launch->key == key && p0.lsm_launch->backend == static_cast<int>(backend))
+        return *p0.lsm_launch;
+    auto L = std::make_unique<LsmLaunch>();
+    L->key = key;
+    L->backend = static_cast<int>(backend);
+    const size_t real = backend == PE_BACKEND_GPU_FP64 ? sizeof(double) : sizeof(float);
+    std::vector<LsmDate> dates;
+    std::vector<uint32_t> sobol;
+    uint64_t store_elems = 0, cash_elems = 0;
+    uint32_t partials = 0;
+    L->basis = plans[0]->lsm_trade.basis;
+    for (size_t ti = 0; ti < plans.size(); ++ti) {
+        HostPlan& p = *plans[ti];
+        if (!p.american) throw PlanError(PE_ERR_INVALID_ARGUMENT, "not an early-exercise plan");
+        if (p.req.lsm.method == PE_LSM_MLP) throw PlanError(PE_ERR_UNSUPPORTED, "MLP policy in a polynomial batch");
+        if (p.lsm_trade.basis != L->basis) throw PlanError(PE_ERR_INVALID_ARGUMENT, "one LSM basis per launch");
+        LsmTrade t = p.lsm_trade;
+        const uint32_t n = t.n_dates;
+        t.date_offset = static_cast<uint32_t>(dates.size());
+        dates.insert(dates.end(), p.lsm_dates.begin(), p.lsm_dates.end());
+        t.store_offset = store_elems;
+        t.cash_offset = cash_elems;
+        store_elems += t.train_paths * n;
+        cash_elems += t.train_paths;
+        t.decision_offset = static_cast<uint32_t>(L->n_decisions);
+        t.count_offset = static_cast<uint32_t>(L->n_counts);
+        L->n_decisions += n - 1;
+        L->n_counts += n;
+        for (LsmEngine* e : {&t.train, &t.value}) {
+            if (!e->rqmc) continue;
+            e->sobol_offset = static_cast<uint32_t>(sobol.size());
+            const std::vector<uint32_t> tab = build_sobol_table(e->seed, e->scrambles, n);
+            if (sobol.size() + tab.size() > 0xffffffffull) throw PlanError(PE_ERR_SIZE_LIMIT, "Sobol tables");
+            sobol.insert(sobol.end(), tab.begin(), tab.end());
+        }
+        L->trades.push_back(t);
+        auto add_items = [&](const LsmEngine& e, std::vector<LsmItem>& items, bool value) {
+            const uint32_t ch = static_cast<uint32_t>((e.units + kItemUnits - 1) / kItemUnits);
+            if (value) {
+                L->first_partial.push_back(partials);
+                L->chunks.push_back(ch);
+                L->scrambles.push_back(e.scrambles);
+            }
+            for (uint32_t sc = 0; sc < e.scrambles; ++sc)
+                for (uint32_t c = 0; c < ch; ++c) {
+                    LsmItem it{};
+                    it.trade = static_cast<uint32_t>(ti);
+                    it.scramble = sc;
+                    it.unit_begin = static_cast<uint64_t>(c) * kItemUnits;
+                    it.unit_end = std::min<uint64_t>(it.unit_begin + kItemUnits, e.units);
+                    it.partial = value ? partials++ : 0u;
+                    items.push_back(it);
+                }
+        };
+        add_items(t.train, L->train_items, false);
+        add_items(t.value, L->value_items, true);
+    }
+    // longest work first
+    auto by_cost = [&](std::vector<LsmItem>& items) {
+        std::stable_sort(items.begin(), items.end(), [&](const LsmItem& x, const LsmItem& y) {
+            return (x.unit_end - x.unit_begin) * L->trades[x.trade].n_dates >
+                   (y.unit_end - y.unit_begin) * L->trades[y.trade].n_dates;
+        });
+        if (items.size() > 0x7fffffffu) throw PlanError(PE_ERR_SIZE_LIMIT, "too many work items");
+    };
+    by_cost(L->train_items);
+    by_cost(L->value_items);
+    L->n_partials = partials;
+    L->d_trades.upload(L->trades);
+    L->d_dates.upload(dates);
+    if (!sobol.empty()) L->d_sobol.upload(sobol);
+    L->d_train_items.upload(L->train_items);
+    L->d_value_items.upload(L->value_items);
+    L->store.reserve(static_cast<size_t>(store_elems) * real + 256);
+    L->cash.reserve(static_cast<size_t>(cash_elems) * sizeof(double) + 256);
+    L->decisions.reserve(std::max<size_t>(1, L->n_decisions) * sizeof(LsmDecision));
+    L->counts.reserve(L->n_counts * sizeof(unsigned long long));
+    L->in_sample.reserve(plans.size() * sizeof(double));
+    L->partials.reserve(std::max<size_t>(1, L->n_partials) * sizeof(BlockStat));
+    L->err.reserve(sizeof(unsigned long long));
+    L->host_partials.resize(L->n_partials);
+    L->host_decisions.resize(L->n_decisions);
+    L->host_counts.resize(L->n_counts);
+    L->host_in_sample.resize(plans.size());
+    for (hipEvent_t& e : L->ev) check(hipEventCreate(&e), "hipEventCreate");
+    p0.lsm_launch.reset(L.release());
+    return *p0.lsm_launch;
+}
+
+void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* outs, pe_lsm_report* reports) {
+    LsmLaunch& L = prepare_lsm(plans, backend);
+    const unsigned long long none = ~0ull;
+    check(hipMemcpyAsync(L.err.ptr, &none, sizeof(none), hipMemcpyHostToDevice, g_stream), "error word");
+    check(hipMemsetAsync(L.counts.ptr, 0, L.n_counts * sizeof(unsigned long long), g_stream), "counts");
+    LsmArgs a{};
+    a.trades = L.d_trades.as<LsmTrade>();
+    a.dates = L.d_dates.as<LsmDate>();
+    a.sobol = L.d_sobol.as<uint32_t>();
+    a.store = L.store.ptr;
+    a.cash = L.cash.as<double>();
+    a.decisions = L.decisions.as<LsmDecision>();
+    a.counts = L.counts.as<unsigned long long>();
+    a.in_sample = L.in_sample.as<double>();
+    a.partials = L.partials.ptr;
+    a.error = L.err.as<unsigned long long>();
+    a.n_trades = static_cast<uint32_t>(plans.size());
+    LsmArgs train = a, value = a;
+    train.items = L.d_train_items.as<LsmItem>();
+    train.n_items = static_cast<uint32_t>(L.train_items.size());
+    value.items = L.d_value_items.as<LsmItem>();
+    value.n_items = static_cast<uint32_t>(L.value_items.size());
+    check(hipEventRecord(L.ev[0], g_stream), "hipEventRecord");
+    check(backend == PE_BACKEND_GPU_FP64
+              ? launch_lsm_f64(train, train.n_items, value, value.n_items, L.basis, g_stream, L.ev[1])
+              : launch_lsm_f32(train, train.n_items, value, value.n_items, L.basis, g_stream, L.ev[1]),
+          "LSM kernel launch");
+    check(hipEventRecord(L.ev[2], g_stream), "hipEventRecord");
+    check(hipEventSynchronize(L.ev[2]), "hipEventSynchronize");
+    float train_ms = 0.0f, value_ms = 0.0f;
+    check(hipEventElapsedTime(&train_ms, L.ev[0], L.ev[1]), "hipEventElapsedTime");
+    check(hipEventElapsedTime(&value_ms, L.ev[1], L.ev[2]), "hipEventElapsedTime");
+    check(hipMemcpy(L.host_partials.data(), L.partials.ptr, L.n_partials * sizeof(BlockStat), hipMemcpyDeviceToHost),
+          "partials copy");
+    if (L.n_decisions)
+        check(hipMemcpy(L.host_decisions.data(), L.decisions.ptr, L.n_decisions * sizeof(LsmDecision),
+                        hipMemcpyDeviceToHost),
+              "decisions copy");
+    check(hipMemcpy(L.host_counts.data(), L.counts.ptr, L.n_counts * sizeof(unsigned long long),
+                    hipMemcpyDeviceToHost),
+          "counts copy");
+    check(hipMemcpy(L.host_in_sample.data(), L.in_sample.ptr, plans.size() * sizeof(double), hipMemcpyDeviceToHost),
+          "in-sample copy");
+    unsigned long long err = none;
+    check(hipMemcpy(&err, L.err.ptr, sizeof(err), hipMemcpyDeviceToHost), "error copy");
+    if (err != none) {
+        const uint64_t trade = err >> 40, unit = err & ((1ull << 40) - 1);
+        throw PlanError(PE_ERR_NONFINITE_STATE, "trade " + std::to_string(trade) + ", sampling unit " +
+                                                    std::to_string(unit) + ": non-finite early-exercise path");
+    }
+    for (size_t ti = 0; ti < plans.size(); ++ti) {
+        const HostPlan& p = *plans[ti];
+        const LsmTrade& t = L.trades[ti];
+        const uint32_t per = L.scrambles[ti] * L.chunks[ti];
+        std::vector<BlockStat> mine(L.host_partials.begin() + L.first_partial[ti],
+                                    L.host_partials.begin() + L.first_partial[ti] + per);
+        pe_result* out = outs + ti;
+        std::memset(out, 0, sizeof(*out));
+        assemble(p, mine, L.chunks[ti], L.scrambles[ti], 1, 0, 1, out);
+        out->device_ms = train_ms + value_ms;
+        if (!reports) continue;
+        pe_lsm_report& r = reports[ti];
+        r.in_sample_value = L.host_in_sample[ti];
+        r.training_ms = train_ms;
+        r.valuation_ms = value_ms;
+        r.exercise_dates = t.n_dates;
+        for (uint32_t d = 0; d < t.n_dates && d < r.capacity; ++d) {
+            if (r.exercise_counts) r.exercise_counts[d] = L.host_counts[t.count_offset + d];
+            if (!r.decisions || d + 1 >= t.n_dates) continue;
+            const LsmDecision& s = L.host_decisions[t.decision_offset + d];
+            pe_lsm_decision& o = r.decisions[d];
+            std::memset(&o, 0, sizeof(o));
+            o.continue_all = s.continue_all ? 1 : 0;
+            o.itm_rows = s.itm_rows;
+            if (s.continue_all) continue;
+            o.rank = s.rank;
+            for (int k = 0; k < 4; ++k) {
+                o.coefficients[k] = s.coef[k];
+                o.diagonal_abs[k] = s.diag[k];
+                o.pivot_order[k] = s.pivot[k];
+            }
+            o.feature_mean = s.mean;
+            o.feature_population_variance = s.variance;
+            o.feature_scale = s.scale;
+            o.feature_inactive = s.inactive ? 1 : 0;
+            o.rank_threshold = s.rank_threshold;
+            o.residual_sum_squares = s.rss;
+        }
+    }
+    g_kernel_ms = train_ms + value_ms;
+    g_kernel_launches = 3;
+}
+
 template <typename F>
 pe_status guarded(F&& f) {
     try {
@@ -845,6 +1062,9 @@ pe_status pe_plan_evaluate(const pe_plan* plan, pe_backend backend, pe_result* o
         if (p.rough) {
             std::vector<HostPlan*> plans{&p};
             evaluate_rough(plans, backend, out);
+        } else if (p.american) {
+            std::vector<HostPlan*> plans{&p};
+            evaluate_lsm(plans, backend, out, nullptr);
         } else if (p.lv) {
             std::vector<HostPlan*> plans{&p};
             evaluate_lv(plans, backend, out);
@@ -901,7 +1121,45 @@ pe_status pe_batch_evaluate(const pe_plan* const* plans, uint32_t count, pe_back
             if (!plans[i]) throw PlanError(PE_ERR_INVALID_ARGUMENT, "null plan");
             hp.push_back(&const_cast<pe_plan*>(plans[i])->host);
         }
-        evaluate_bs_batch(hp, backend, results);
+        // American plans by exercise method, the rest through the BS batch
+        std::vector<HostPlan*> groups[3], rest;
+        std::vector<size_t> where[3], rest_at;
+        for (size_t i = 0; i < hp.size(); ++i) {
+            if (hp[i]->american) {
+                const int g = static_cast<int>(hp[i]->req.lsm.method);
+                groups[g].push_back(hp[i]);
+                where[g].push_back(i);
+            } else {
+                rest.push_back(hp[i]);
+                rest_at.push_back(i);
+            }
+        }
+        auto scatter = [&](std::vector<HostPlan*>& ps, const std::vector<size_t>& at, auto&& run) {
+            if (ps.empty()) return;
+            std::vector<pe_result> tmp(ps.size());
+            run(ps, tmp.data());
+            for (size_t k = 0; k < at.size(); ++k) results[at[k]] = tmp[k];
+        };
+        for (int g = 0; g < 3; ++g)
+            scatter(groups[g], where[g], [&](std::vector<HostPlan*>& ps, pe_result* o) {
+                evaluate_lsm(ps, backend, o, nullptr);
+            });
+        scatter(rest, rest_at, [&](std::vector<HostPlan*>& ps, pe_result* o) { evaluate_bs_batch(ps, backend, o); });
+    });
+}
+
+pe_status pe_lsm_evaluate(const pe_plan* const* plans, uint32_t count, pe_backend backend, pe_result* results,
+                          pe_lsm_report* reports) {
+    if (!plans || !results || count == 0) return fail(PE_ERR_INVALID_ARGUMENT, "null argument");
+    if (backend != PE_BACKEND_GPU_FP64 && backend != PE_BACKEND_GPU_MIXED) return fail(PE_ERR_INVALID_ARGUMENT, "backend");
+    return guarded([&] {
+        ensure_init();
+        std::vector<HostPlan*> hp;
+        for (uint32_t i = 0; i < count; ++i) {
+            if (!plans[i]) throw PlanError(PE_ERR_INVALID_ARGUMENT, "null plan");
+            hp.push_back(&const_cast<pe_plan*>(plans[i])->host);
+        }
+        evaluate_lsm(hp, backend, results, reports);
     });
 }
 

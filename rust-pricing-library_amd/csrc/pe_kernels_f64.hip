@@ -3,6 +3,7 @@
 #include "pe_kernels.inc"
 #include "pe_bs.inc"
 #include "pe_rough.inc"
+#include "pe_lsm.inc"
 #include "pe_plan.hpp"
 
 namespace pe {
@@ -14,6 +15,11 @@ hipError_t launch_bs_batch_f64(const BsBatchArgs& a, bool risk, bool cont, hipSt
 hipError_t launch_rough_f64(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 grid, uint32_t payoff_items,
                             unsigned long long* error, hipStream_t s) {
     return launch_rough<double>(a, pa, grid, payoff_items, error, s);
+}
+
+hipError_t launch_lsm_f64(const LsmArgs& train, uint32_t train_items, const LsmArgs& value, uint32_t value_items,
+                          uint32_t basis, hipStream_t s, hipEvent_t between) {
+    return launch_lsm<double>(train, train_items, value, value_items, basis, s, between);
 }
 
 hipError_t launch_lv_f64(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s) {

@@ -63,6 +63,7 @@ std::vector<int32_t> fixing_days(const pe_product& p) {
                 for (uint32_t i = 0; i < p.date_count; ++i) add(p.dates[i]);
             break;
         case PE_PRODUCT_LOOKBACK:
+        case PE_PRODUCT_AMERICAN:
             for (uint32_t i = 0; i < p.date_count; ++i) add(p.dates[i]);
             break;
         default:
@@ -518,6 +519,82 @@ void compile_rough_plan(HostPlan& p, const pe_request& carrier, const pe_lifted_
 }
 
 // ------------------------------------------------------------------ plan --
+// Early-exercise runtime (execute_early_exercise, crates/pricing/src/engine/risk/early_exercise.rs:24-300):
+// per exercise date the exact-step constants and DF(date); the training and
+// valuation engines and the regression settings of the LsmConfig.
+void compile_lsm_trade(HostPlan& p) {
+    const pe_request& r = p.req;
+    const pe_lsm& l = r.lsm;
+    if (p.lv) unsupported("early exercise under local volatility on the device");
+    if (p.risk) unsupported("fixed-policy early-exercise Greeks");
+    if (l.method != PE_LSM_POLY_CPQR && l.method != PE_LSM_POLY_LAGUERRE && l.method != PE_LSM_MLP)
+        invalid("LSM method");
+    if (l.max_degree + 1 > static_cast<uint32_t>(kLsmMaxBasis)) unsupported("LSM basis degree above 3 on the device");
+    const pe_engine& te = l.training_engine;
+    const pe_engine& ve = r.engine;
+    if (te.kind != ve.kind) invalid("training and valuation engines must be of one kind");
+    if (te.units == 0) invalid("zero training units");
+    if (te.kind == PE_ENGINE_RQMC && (te.scrambles == 0 || te.units > 0xffffffffull))
+        invalid("RQMC needs scrambles and at most 2^32 points");
+    if (!std::isfinite(l.abs_rank_tolerance) || l.abs_rank_tolerance < 0.0 || !std::isfinite(l.rel_rank_tolerance) ||
+        l.rel_rank_tolerance < 0.0 || !std::isfinite(l.itm_abs_tolerance) || l.itm_abs_tolerance < 0.0)
+        invalid("LSM tolerances must be finite and non-negative");
+    const size_t n = p.obs_times.size();
+    if (n == 0 || n > static_cast<size_t>(kLsmMaxDates)) throw PlanError(PE_ERR_SIZE_LIMIT, "exercise date count");
+    const uint64_t train_mult = te.antithetic ? 2 : 1;
+    const uint64_t train_paths = te.units * (te.kind == PE_ENGINE_RQMC ? te.scrambles : 1u) * train_mult;
+    if (l.max_matrix_elements && train_paths > l.max_matrix_elements)
+        invalid("regression matrix element limit");
+    p.lsm_dates.assign(n, LsmDate{});
+    for (size_t i = 0; i < n; ++i) {
+        LsmDate& d = p.lsm_dates[i];
+        const BsObs& o = p.bs_obs[i];
+        d.time = o.time;
+        d.sqrt_step = o.sqrt_step;
+        d.forward = o.forward;
+        d.discount = std::exp(p.discount_curve.log_discount(static_cast<double>(p.obs_days[i]) / 365.0));
+        d.a = o.a;
+        d.b = o.b;
+        d.m_half_tv = -0.5 * (p.vol * p.vol * o.time);
+        d.f_sqrt_step = static_cast<float>(d.sqrt_step);
+        d.f_forward = static_cast<float>(d.forward);
+        d.f_m_half_tv = static_cast<float>(d.m_half_tv);
+    }
+    LsmTrade& t = p.lsm_trade;
+    std::memset(&t, 0, sizeof(t));
+    auto engine = [&](const pe_engine& e, uint32_t domain) {
+        LsmEngine g{};
+        g.seed = e.seed;
+        g.units = e.units;
+        g.scrambles = e.kind == PE_ENGINE_RQMC ? e.scrambles : 1u;
+        g.rqmc = e.kind == PE_ENGINE_RQMC ? 1u : 0u;
+        g.antithetic = e.antithetic ? 1u : 0u;
+        g.domain = domain;
+        return g;
+    };
+    t.train = engine(te, 1u);   // SamplingDomain::LsmTrain
+    t.value = engine(ve, 0u);   // SamplingDomain::Valuation
+    t.train_paths = train_paths;
+    t.n_dates = static_cast<uint32_t>(n);
+    t.call = p.req.product.side == PE_CALL ? 1u : 0u;
+    t.zero_variance = p.total_variance == 0.0 ? 1u : 0u;
+    t.single_terminal = n == 1 ? 1u : 0u;
+    t.basis = l.method == PE_LSM_POLY_LAGUERRE || l.method == PE_LSM_MLP ? kLsmLaguerre : kLsmMonomial;
+    t.columns = l.max_degree + 1;
+    t.strike = p.req.product.strike;
+    t.notional = p.req.product.notional;
+    t.s0 = p.spot;
+    t.vol = p.vol;
+    t.abs_tol = l.abs_rank_tolerance;
+    t.rel_tol = l.rel_rank_tolerance;
+    t.itm_tol = l.itm_abs_tolerance;
+    t.inv_strike = 1.0 / t.strike;
+    t.f_vol = static_cast<float>(p.vol);
+    t.f_strike = static_cast<float>(t.strike);
+    t.f_s0 = static_cast<float>(p.spot);
+    t.f_inv_strike = static_cast<float>(t.inv_strike);
+}
+
 void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* policy) {
     static std::atomic<uint64_t> next_id{1};
     p.id = next_id.fetch_add(1);
@@ -660,6 +737,14 @@ void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* p
             for (uint32_t i = 0; i < pr.date_count; ++i)
                 if (pr.dates[i] >= 0) p.obs_weight[obs_index(pr.dates[i])] = 1.0;
             break;
+        case PE_PRODUCT_AMERICAN:
+            // exercise dates are the observations (execute_early_exercise needs
+            // every date in the future: discount factors per date)
+            p.american = true;
+            d.kind = kEuropean;
+            for (uint32_t i = 0; i < pr.date_count; ++i)
+                if (pr.dates[i] < 0) invalid("exercise dates must be future and distinct");
+            break;
         default:
             unsupported("product kind");
     }
@@ -759,6 +844,7 @@ void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* p
     p.vol_bump = p.vol == 0.0 ? 0.0 : std::min(kVolatilityBump, p.vol * 0.5);
     p.sobol_dims = p.lv ? static_cast<uint32_t>(p.nodes.size() - 1) : static_cast<uint32_t>(p.obs_times.size());
     if (!p.lv) compile_bs_trade(p, monitoring_end);
+    if (p.american) compile_lsm_trade(p);
 }
 
 }  // namespace pe

@@ -120,6 +120,10 @@ struct RoughLaunch;
 struct RoughLaunchDeleter {
     void operator()(RoughLaunch* l) const;
 };
+struct LsmLaunch;
+struct LsmLaunchDeleter {
+    void operator()(LsmLaunch* l) const;
+};
 
 struct HostPlan {
     uint64_t id = 0;                 // unique per compiled plan, never reused
@@ -174,6 +178,12 @@ struct HostPlan {
     BsTrade trade{};
     std::vector<BsObs> bs_obs;
 
+    // early exercise (American product): exercise dates and the LSM runtime
+    bool american = false;
+    LsmTrade lsm_trade{};
+    std::vector<LsmDate> lsm_dates;
+    std::unique_ptr<LsmLaunch, LsmLaunchDeleter> lsm_launch;
+
     // device tables
     DeviceBuffer d_obs, d_sobol, d_steps, d_program, d_grid, d_grid32, d_x, d_x32, d_node_weight;
 
@@ -201,6 +211,10 @@ hipError_t launch_rough_f64(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 
                             unsigned long long* error, hipStream_t s);
 hipError_t launch_rough_f32(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 grid, uint32_t payoff_items,
                             unsigned long long* error, hipStream_t s);
+hipError_t launch_lsm_f64(const LsmArgs& train, uint32_t train_items, const LsmArgs& value, uint32_t value_items,
+                          uint32_t basis, hipStream_t s, hipEvent_t between);
+hipError_t launch_lsm_f32(const LsmArgs& train, uint32_t train_items, const LsmArgs& value, uint32_t value_items,
+                          uint32_t basis, hipStream_t s, hipEvent_t between);
 hipError_t launch_lv_f64(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s);
 hipError_t launch_lv_f32(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s);
 hipError_t launch_dupire(const double* theta_t, const double* theta_v, const double* theta_d, uint32_t nk,

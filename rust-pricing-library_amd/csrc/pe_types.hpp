@@ -231,6 +231,79 @@ struct RoughPayoffArgs {
     uint32_t pad;
 };
 
+// ------------------------------------------------------- early exercise --
+constexpr int kLsmThreads = 1024;     // induction workgroup: one per trade
+constexpr int kLsmMaxBasis = 4;       // spot feature, degree <= 3
+constexpr int kLsmMaxDates = 1024;    // exercise dates per trade
+
+enum LsmBasisKind : uint32_t {
+    kLsmMonomial = 0,   // standardized monomials, CPQR pivot/rank rules, all-path decisions
+    kLsmLaguerre = 1,   // exp(-x/2) L_k(x), x = S / K, in-the-money decisions
+};
+
+// One exercise date of a trade: exact BS/Black-76 step and the escrowed
+// dividend coordinate at the initial spot, S = a * S0 + b * F.
+struct LsmDate {
+    double time, sqrt_step, forward, discount;   // discount = DF(date)
+    double a, b;
+    double m_half_tv;                            // -vol^2 t / 2
+    float f_sqrt_step, f_forward, f_m_half_tv, f_pad;
+};
+
+// A sampling engine of a trade (training or valuation).
+struct LsmEngine {
+    uint64_t seed, units;         // pseudo units or RQMC points per scramble
+    uint32_t scrambles, rqmc, antithetic, domain;
+    uint32_t sobol_offset, pad;   // words into the batch Sobol table
+};
+
+struct LsmTrade {
+    LsmEngine train, value;
+    uint64_t store_offset;        // store elements before this trade ([date][path])
+    uint64_t cash_offset;         // training paths before this trade
+    uint64_t train_paths;         // units * scrambles * (1 + antithetic)
+    uint32_t date_offset, n_dates;
+    uint32_t decision_offset;     // non-terminal dates before this trade
+    uint32_t count_offset;        // dates before this trade
+    uint32_t call, zero_variance, single_terminal, basis;
+    uint32_t columns;             // max degree + 1
+    uint32_t pad;
+    double strike, notional, s0, vol;
+    double abs_tol, rel_tol, itm_tol, inv_strike;
+    float f_vol, f_strike, f_s0, f_inv_strike;
+};
+
+// One date's fitted continuation model (ExerciseDecisionModel).
+struct LsmDecision {
+    double coef[kLsmMaxBasis];    // per basis column, original order
+    double diag[kLsmMaxBasis];    // |R_pp| in pivot order
+    double mean, variance, scale, inv_scale;   // monomial feature standardisation
+    double rank_threshold, rss;
+    uint32_t pivot[kLsmMaxBasis];
+    uint32_t rank, continue_all, inactive, itm_rows;
+};
+
+struct LsmItem {
+    uint32_t trade, scramble;
+    uint64_t unit_begin, unit_end;
+    uint32_t partial, pad;
+};
+
+struct LsmArgs {
+    const LsmTrade* trades;
+    const LsmDate* dates;
+    const uint32_t* sobol;
+    const LsmItem* items;
+    void* store;                      // training spots, Real [date][path] per trade
+    double* cash;                     // training cash flow x DF(stopping date)
+    LsmDecision* decisions;
+    unsigned long long* counts;       // valuation exercise counts [trade][date]
+    double* in_sample;                // per trade: mean of the training cash flows
+    void* partials;                   // BlockStat per valuation item
+    unsigned long long* error;
+    uint32_t n_items, n_trades;
+};
+
 // Kernel arguments of the local-vol Log-Euler kernel.
 struct LvArgs {
     const LvStep* steps;

@@ -240,6 +240,36 @@ def lifted_heston(v0, kappa, theta, nu, rho, weights, rates):
     return m
 
 
+class pe_lsm_decision(C.Structure):
+    _fields_ = [
+        ("continue_all", C.c_int32),
+        ("rank", C.c_uint32),
+        ("itm_rows", C.c_uint64),
+        ("coefficients", C.c_double * 4),
+        ("diagonal_abs", C.c_double * 4),
+        ("pivot_order", C.c_uint32 * 4),
+        ("feature_mean", C.c_double),
+        ("feature_population_variance", C.c_double),
+        ("feature_scale", C.c_double),
+        ("feature_inactive", C.c_int32),
+        ("pad", C.c_int32),
+        ("rank_threshold", C.c_double),
+        ("residual_sum_squares", C.c_double),
+    ]
+
+
+class pe_lsm_report(C.Structure):
+    _fields_ = [
+        ("in_sample_value", C.c_double),
+        ("training_ms", C.c_double),
+        ("valuation_ms", C.c_double),
+        ("exercise_dates", C.c_uint32),
+        ("capacity", C.c_uint32),
+        ("exercise_counts", C.POINTER(C.c_uint64)),
+        ("decisions", C.POINTER(pe_lsm_decision)),
+    ]
+
+
 PE_SYMBOLS = {
     "pe_init": (C.c_int, [C.c_int32]),
     "pe_last_error": (C.c_char_p, []),
@@ -257,6 +287,8 @@ PE_SYMBOLS = {
         [C.POINTER(pe_ssvi_surface), C.c_uint32, _DP, C.c_uint32, _DP, C.c_double, C.c_double, _DP,
          C.POINTER(C.c_uint32)],
     ),
+    "pe_lsm_evaluate": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.c_int, C.POINTER(pe_result),
+                                  C.POINTER(pe_lsm_report)]),
     "pe_last_kernel_stats": (C.c_int, [_DP, C.POINTER(C.c_uint32)]),
 }
 

@@ -128,6 +128,38 @@ def evaluate_batch(plans, backend="fp64"):
     return list(results)
 
 
+class LsmReport:
+    """Early-exercise diagnostics of one plan: in-sample value, valuation
+    exercise counts, the decision model per non-terminal date, device ms."""
+
+    def __init__(self, r, counts, decisions):
+        self.in_sample_value = r.in_sample_value
+        self.training_ms = r.training_ms
+        self.valuation_ms = r.valuation_ms
+        self.exercise_dates = r.exercise_dates
+        self.exercise_counts = list(counts)[:r.exercise_dates]
+        self.decisions = list(decisions)[:max(r.exercise_dates - 1, 0)]
+
+
+def evaluate_lsm(plans, backend="fp64", max_dates=1024):
+    """Prices American plans by least-squares Monte Carlo on the device;
+    returns (results, reports)."""
+    n = len(plans)
+    handles = (C.c_void_p * n)(*[p._handle.value for p in plans])
+    results = (abi.pe_result * n)()
+    reports = (abi.pe_lsm_report * n)()
+    keep = []
+    for r in reports:
+        counts = (C.c_uint64 * max_dates)()
+        decisions = (abi.pe_lsm_decision * max_dates)()
+        r.capacity = max_dates
+        r.exercise_counts = C.cast(counts, C.POINTER(C.c_uint64))
+        r.decisions = C.cast(decisions, C.POINTER(abi.pe_lsm_decision))
+        keep.append((counts, decisions))
+    _check(load().pe_lsm_evaluate(handles, n, BACKENDS[backend], results, reports))
+    return list(results), [LsmReport(r, c, d) for r, (c, d) in zip(reports, keep)]
+
+
 def local_variance_from_ssvi(theta_times, theta_values, terminal_slope, rho, eta, gamma, time_nodes, x_nodes,
                              floor, cap):
     """Dupire local variance of a StandardSsvi surface, built in FP64 on the GPU."""

@@ -213,3 +213,36 @@ def rough_smile(points=1 << 16, scrambles=16, strikes=21, maturities=SMILE_MATUR
             req.set_risk()
             out.append(req)
     return out
+
+
+AMERICAN_RATE = 0.03
+
+
+def american_batch(count=256, seed=0x5EED_1A5E, train_log2=18, value_log2=18, dates=50,
+                   method=abi.PE_LSM_POLY_LAGUERRE, antithetic=False, dividends=False):
+    """American puts priced by LSM: S0 = 100, K ~ U[80, 120], sigma ~ U[0.2,
+    0.4], r = 3%, q = 0, T = 1 year, `dates` exercise dates at round(365 k /
+    dates) days; per trade 2^train_log2 training paths (pseudo-MC, domain
+    LsmTrain) and 2^value_log2 independent valuation paths, seeds per trade;
+    the weighted-Laguerre basis of degree 3 (or the method given).  With
+    `dividends`, one 1.0 cash + 1% dividend at half a year."""
+    rng = SplitMix64(seed)
+    out = []
+    horizon = 1.0
+    days = sorted({max(1, int(round(365.0 * (k + 1) / dates))) for k in range(dates)})
+    for tid in range(count):
+        strike = rng.uniform(80.0, 120.0)
+        vol = rng.uniform(0.2, 0.4)
+        seed_t, seed_v = rng.next(), rng.next()
+        req = request.Request()
+        divs = [(182 / 365.0, 1.0, 0.01)] if dividends else []
+        req.set_market(100.0, [0.0, horizon], [1.0, math.exp(-AMERICAN_RATE * horizon)], [0.0, horizon], [1.0, 1.0],
+                       divs)
+        req.set_black_scholes(vol)
+        req.set_product(abi.PE_PRODUCT_AMERICAN, abi.PE_PUT, days[-1], strike, dates=days)
+        req.set_pseudo(seed_v, 1 << value_log2, antithetic=antithetic)
+        req.set_risk()
+        req.set_lsm((abi.PE_ENGINE_PSEUDO, seed_t, 1 << train_log2, 0), antithetic=antithetic, method=method)
+        req.trade_id = tid
+        out.append(req)
+    return out

@@ -72,6 +72,9 @@ def oracle():
     lib.oracle_cpqr.argtypes = [D, C.c_uint32, C.c_uint32, D, C.c_double, C.c_double, D]
     lib.oracle_basis_exponents.argtypes = [C.c_uint32, C.c_uint32, U, C.c_uint32]
     lib.oracle_feature_scaling.argtypes = [D, C.c_uint32, D]
+    lib.oracle_cholesky_fit.argtypes = [D, C.c_uint32, C.c_uint32, D, C.c_double, C.c_double, D]
+    lib.oracle_cpqr_diagonals.argtypes = [D, C.c_uint32, C.c_uint32, D, C.c_double, C.c_double, D]
+    lib.oracle_weighted_laguerre.argtypes = [C.c_double, C.c_uint32, D]
     lib.oracle_lifted_from_rough.argtypes = [C.c_double, C.c_uint32, C.c_double, D, D]
     lib.oracle_lifted_path.argtypes = [C.POINTER(abi.pe_lifted_heston), D, C.c_uint32, C.c_double, D, D, D]
     lib.oracle_rough_evaluate.argtypes = [C.POINTER(C.POINTER(abi.pe_request)), C.c_uint32,
@@ -118,3 +121,18 @@ def oracle_rough(reqs, model, maximum_step, workers=4, block=0):
     if rc != 0:
         raise RuntimeError(f"oracle_rough_evaluate failed ({rc}): {lib.oracle_last_error().decode()}")
     return list(out), nodes.value
+
+
+def oracle_lsm(req, block=256, workers=4):
+    """The oracle's early-exercise evaluation: (result, per-date decision
+    rows of 20 values, in-sample value, valuation exercise counts)."""
+    from pricing_amd import abi, request
+    res = abi.pe_result()
+    diag = (C.c_double * (20 * 1024))()
+    ins = C.c_double()
+    counts = (C.c_uint64 * 1024)()
+    pol = request.exec_policy(workers, block)
+    n = oracle().oracle_lsm_evaluate(req.pointer(), C.byref(pol), C.byref(res), diag, 1024, C.byref(ins), counts)
+    if n < 0:
+        raise RuntimeError(oracle().oracle_last_error().decode())
+    return res, [list(diag[20 * d:20 * d + 20]) for d in range(n)], ins.value, list(counts)

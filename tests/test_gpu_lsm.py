@@ -1,0 +1,185 @@
+"""GPU parity of least-squares Monte Carlo (American / Bermudan exercise):
+training paths, backward induction and out-of-sample valuation on the device
+against the CPU oracle and the reference's frozen American put replays
+(fixtures/replay/early_exercise-windows-x86_64.json).
+
+Bars: the FP64 device mode reproduces the oracle's price to 1e-12 relative,
+its exercise counts exactly and its in-sample value to 1e-12; decision models
+agree to the conditioning of the normal equations (the device solves the
+Gram system with CPQR's pivot and rank rules, the oracle runs the reference's
+Householder QR).  The mixed mode stays within 0.1 standard errors of FP64.
+"""
+import math
+
+import pytest
+
+from _support import golden, oracle_lsm
+from pricing_amd import abi, engine, request, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _init():
+    engine.init(0)
+
+
+REPLAY = golden()["early_exercise_replay"]
+
+
+def _device(req, backend="fp64"):
+    plan = engine.PricingPlan.compile(req)
+    res, rep = engine.evaluate_lsm([plan], backend)
+    return res[0], rep[0]
+
+
+def _laguerre_prediction(coef, spot, strike):
+    import ctypes as C
+    from _support import oracle
+    phi = (C.c_double * 4)()
+    oracle().oracle_weighted_laguerre(spot / strike, 3, phi)
+    return sum(a * b for a, b in zip(phi, coef))
+
+
+def _compare_oracle(req, block=256):
+    laguerre = req.c.lsm.method == abi.PE_LSM_POLY_LAGUERRE
+    strike = req.c.product.strike
+    dev, rep = _device(req)
+    ref, diag, ins, counts = oracle_lsm(req, block)
+    n = rep.exercise_dates
+    assert rep.exercise_counts == counts[:n]
+    assert abs(dev.estimates[0].value - ref.estimates[0].value) <= 1e-12 * abs(ref.estimates[0].value)
+    assert abs(dev.estimates[0].standard_error - ref.estimates[0].standard_error) <= \
+        1e-10 * ref.estimates[0].standard_error
+    assert abs(rep.in_sample_value - ins) <= 1e-12 * abs(ins)
+    for d, row in enumerate(diag):
+        m = rep.decisions[d]
+        assert m.continue_all == int(row[19]) and m.itm_rows == int(row[0]), d
+        if m.continue_all:
+            continue
+        assert m.rank == int(row[1]), d
+        assert list(m.pivot_order)[:m.rank] == [int(v) for v in row[14:14 + m.rank]], d
+        if laguerre:
+            # the weighted Laguerre Gram is ill-conditioned on the in-the-money
+            # range (condition ~1e9: the summation order alone moves the fit
+            # by ~1e-8): compare the fitted continuation inside the data range
+            for spot in (70.0, 80.0, 90.0, 95.0, 99.0):
+                a = _laguerre_prediction(m.coefficients, spot, strike)
+                b = _laguerre_prediction(row[6:10], spot, strike)
+                assert abs(a - b) <= 1e-6 * max(1.0, abs(b)), (d, spot, a, b)
+            for k in range(4):
+                assert abs(m.diagonal_abs[k] - row[10 + k]) <= 1e-7 * row[10], (d, k)
+            continue
+        for k in range(4):
+            assert abs(m.coefficients[k] - row[6 + k]) <= 1e-8 * max(1.0, abs(row[6 + k])), (d, k)
+            assert abs(m.diagonal_abs[k] - row[10 + k]) <= 1e-9 * max(row[10 + k], 1e-300), (d, k)
+    return dev, rep
+
+
+@pytest.mark.parametrize("case", REPLAY, ids=[c["name"] for c in REPLAY])
+def test_replay_on_device(case):
+    req = request.Request.from_v3(case["request"])
+    req.set_risk()   # fixed-policy Greeks are not on the device; the price is the replay's
+    dev, rep = _device(req)
+    exp = case["value"]
+    assert abs(dev.estimates[0].value - exp["value"]) <= 1e-12 * abs(exp["value"])
+    assert abs(dev.estimates[0].standard_error - exp["standard_error"]) <= 1e-10 * exp["standard_error"]
+    ee = case["early_exercise"]
+    assert rep.exercise_counts[:len(ee["exercise_counts"])] == ee["exercise_counts"]
+    assert abs(rep.in_sample_value - ee["in_sample_value"]) <= 1e-12 * abs(ee["in_sample_value"])
+    for d, model in enumerate(ee["decision_models"]):
+        m = model["model"]
+        got = rep.decisions[d]
+        assert got.rank == m["rank"]
+        assert list(got.pivot_order) == m["pivot_order"]
+        for k in range(4):
+            assert abs(got.coefficients[k] - m["coefficients"][k]) <= 1e-9 * max(1.0, abs(m["coefficients"][k]))
+            assert abs(got.diagonal_abs[k] - m["diagonal_abs"][k]) <= 1e-10 * m["diagonal_abs"][k]
+        sc = m["feature_scalings"][0]
+        assert abs(got.feature_mean - sc["mean"]) <= 1e-13 * abs(sc["mean"])
+        assert abs(got.feature_population_variance - sc["population_variance"]) <= 1e-11 * sc["population_variance"]
+        assert abs(got.residual_sum_squares - m["residual_sum_squares"]) <= 1e-8 * m["residual_sum_squares"]
+
+
+def _put(method, units, training_units, vol=0.25, rqmc=False, divs=(), antithetic=True, dates=None, strike=100.0):
+    req = request.Request().set_market(100.0, [0.0, 1.0], [1.0, math.exp(-0.05)], [0.0, 1.0], [1.0, 1.0],
+                                       list(divs))
+    req.set_black_scholes(vol)
+    dates = dates or [73, 146, 219, 292, 365]
+    req.set_product(abi.PE_PRODUCT_AMERICAN, abi.PE_PUT, dates[-1], strike, dates=dates)
+    if rqmc:
+        req.set_rqmc(0xFEDCBA9876543210, units, 4, antithetic=antithetic)
+        req.set_lsm((abi.PE_ENGINE_RQMC, 0x1234, training_units, 4), antithetic=antithetic, method=method)
+    else:
+        req.set_pseudo(8738, units, antithetic=antithetic)
+        req.set_lsm((abi.PE_ENGINE_PSEUDO, 4369, training_units, 0), antithetic=antithetic, method=method)
+    req.set_risk()
+    return req
+
+
+METHODS = {"cpqr": abi.PE_LSM_POLY_CPQR, "laguerre": abi.PE_LSM_POLY_LAGUERRE}
+MONTHLY = [30, 61, 91, 122, 152, 182, 213, 243, 274, 304, 335, 365]
+
+
+@pytest.mark.parametrize("method", sorted(METHODS))
+@pytest.mark.parametrize("shape", ["pseudo", "rqmc", "dividends", "monthly"])
+def test_fp64_matches_oracle(method, shape):
+    kw = {"pseudo": {}, "rqmc": {"rqmc": True}, "dividends": {"divs": [(0.5, 1.0, 0.01)]},
+          "monthly": {"dates": MONTHLY, "antithetic": False}}[shape]
+    req = _put(METHODS[method], 1024, 2048, **kw)
+    print(method, shape, flush=True)
+    _compare_oracle(req)
+
+
+def test_zero_volatility_put_exercises_at_the_first_date():
+    req = request.Request().set_market(100.0, [0.0, 1.0], [1.0, math.exp(-0.05)], [0.0, 1.0], [1.0, 1.0])
+    req.set_black_scholes(0.0)
+    req.set_product(abi.PE_PRODUCT_AMERICAN, abi.PE_PUT, 365, 120.0, dates=[91, 182, 274, 365])
+    req.set_pseudo(1, 64)
+    req.set_risk()
+    for method in METHODS.values():
+        req.set_lsm((abi.PE_ENGINE_PSEUDO, 2, 64, 0), method=method)
+        for backend in ("fp64", "mixed"):
+            dev, rep = _device(req, backend)
+            s1 = 100.0 * math.exp(0.05 * 91 / 365)
+            assert dev.estimates[0].value == pytest.approx((120.0 - s1) * math.exp(-0.05 * 91 / 365), rel=1e-6)
+            assert rep.exercise_counts[0] == 64
+
+
+@pytest.mark.parametrize("method", sorted(METHODS))
+def test_mixed_within_tenth_se(method):
+    req = _put(METHODS[method], 1 << 15, 1 << 15, dates=MONTHLY, antithetic=False)
+    plan = engine.PricingPlan.compile(req)
+    (f64,), _ = engine.evaluate_lsm([plan], "fp64")
+    (mixed,), _ = engine.evaluate_lsm([plan], "mixed")
+    se = f64.estimates[0].standard_error
+    assert abs(mixed.estimates[0].value - f64.estimates[0].value) < 0.1 * se, (
+        mixed.estimates[0].value, f64.estimates[0].value, se)
+
+
+def test_batch_equals_single_plans_and_oracle():
+    reqs = workloads.american_batch(24, seed=5, train_log2=10, value_log2=10, dates=12, dividends=True)
+    plans = [engine.PricingPlan.compile(r) for r in reqs]
+    batch, reps = engine.evaluate_lsm(plans, "fp64")
+    via_batch_api = engine.evaluate_batch(plans, "fp64")
+    for i in (0, 5, 17):
+        single, rep = engine.evaluate_lsm([plans[i]], "fp64")
+        assert single[0].estimates[0].value == batch[i].estimates[0].value
+        assert rep[0].exercise_counts == reps[i].exercise_counts
+    for i, r in enumerate(reqs):
+        assert via_batch_api[i].estimates[0].value == batch[i].estimates[0].value
+        if i % 6 == 0:
+            ref, _, ins, counts = oracle_lsm(r)
+            assert reps[i].exercise_counts == counts[:12]
+            assert abs(batch[i].estimates[0].value - ref.estimates[0].value) <= 1e-12 * ref.estimates[0].value
+
+
+def test_laguerre_batch_mixed_within_tenth_se():
+    reqs = workloads.american_batch(16, seed=9, train_log2=14, value_log2=14, dates=50)
+    plans = [engine.PricingPlan.compile(r) for r in reqs]
+    f64, _ = engine.evaluate_lsm(plans, "fp64")
+    mixed, _ = engine.evaluate_lsm(plans, "mixed")
+    for r, a, b in zip(reqs, f64, mixed):
+        se = a.estimates[0].standard_error
+        assert abs(a.estimates[0].value - b.estimates[0].value) < 0.1 * se, (r.trade_id, a.estimates[0].value,
+                                                                             b.estimates[0].value, se)
