@@ -901,7 +901,7 @@ LsmLaunch& prepare_lsm(std::vector<HostPlan*>& plans, pe_backend backend) {
     L->d_train_items.upload(L->train_items);
     L->d_value_items.upload(L->value_items);
     L->store.reserve(static_cast<size_t>(store_elems) * real + 256);
-    L->cash.reserve(static_cast<size_t>(cash_elems) * sizeof(double) + 256);
+    L->cash.reserve(static_cast<size_t>(cash_elems) * real + 256);
     L->decisions.reserve(std::max<size_t>(1, L->n_decisions) * sizeof(LsmDecision));
     L->counts.reserve(L->n_counts * sizeof(unsigned long long));
     L->in_sample.reserve(plans.size() * sizeof(double));
@@ -926,7 +926,7 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
     a.dates = L.d_dates.as<LsmDate>();
     a.sobol = L.d_sobol.as<uint32_t>();
     a.store = L.store.ptr;
-    a.cash = L.cash.as<double>();
+    a.cash = L.cash.ptr;
     a.decisions = L.decisions.as<LsmDecision>();
     a.counts = L.counts.as<unsigned long long>();
     a.in_sample = L.in_sample.as<double>();

@@ -556,6 +556,7 @@ void compile_lsm_trade(HostPlan& p) {
         d.a = o.a;
         d.b = o.b;
         d.m_half_tv = -0.5 * (p.vol * p.vol * o.time);
+        d.inv_discount = 1.0 / d.discount;
         d.f_sqrt_step = static_cast<float>(d.sqrt_step);
         d.f_forward = static_cast<float>(d.forward);
         d.f_m_half_tv = static_cast<float>(d.m_half_tv);

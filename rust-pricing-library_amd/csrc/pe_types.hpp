@@ -247,6 +247,7 @@ struct LsmDate {
     double time, sqrt_step, forward, discount;   // discount = DF(date)
     double a, b;
     double m_half_tv;                            // -vol^2 t / 2
+    double inv_discount;                         // 1 / DF(date) (mixed targets)
     float f_sqrt_step, f_forward, f_m_half_tv, f_pad;
 };
 
@@ -281,6 +282,8 @@ struct LsmDecision {
     double rank_threshold, rss;
     uint32_t pivot[kLsmMaxBasis];
     uint32_t rank, continue_all, inactive, itm_rows;
+    float fcoef[kLsmMaxBasis];    // FP32 copies for the mixed decisions
+    float fmean, finv_scale;
 };
 
 struct LsmItem {
@@ -295,7 +298,7 @@ struct LsmArgs {
     const uint32_t* sobol;
     const LsmItem* items;
     void* store;                      // training spots, Real [date][path] per trade
-    double* cash;                     // training cash flow x DF(stopping date)
+    void* cash;                       // training cash flow x DF(stopping date), Real
     LsmDecision* decisions;
     unsigned long long* counts;       // valuation exercise counts [trade][date]
     double* in_sample;                // per trade: mean of the training cash flows
