@@ -344,6 +344,16 @@ typedef struct pe_lsm_report {
 pe_status pe_lsm_evaluate(const pe_plan* const* plans, uint32_t count, pe_backend backend, pe_result* results,
                           pe_lsm_report* reports);
 
+/* Continuation network of the MLP policy, exposed for verification (no
+ * reference counterpart).  Parameters: 4417 floats, W1[64][2] | b1[64] |
+ * W2[64][64] (row = output unit) | b2[64] | w3[64] | b3; inputs x row-major
+ * rows x 2.  pe_mlp_gradients returns the gradient of the mean squared error
+ * (1/rows) sum (out - y)^2 and the loss, computed by the device's BF16
+ * matrix-core forward and backward passes with FP32 accumulation. */
+pe_status pe_mlp_gradients(const float* params, const float* x, const float* y, uint32_t rows, float* grad_out,
+                           double* loss_out);
+pe_status pe_mlp_forward(const float* params, const float* x, uint32_t rows, float* out);
+
 /* ------------------------------------------------- lifted Heston (rough) -- */
 
 /* LiftedHeston (crates/pricing/src/models/rough_volatility.rs:172-212): a

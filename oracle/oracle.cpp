@@ -37,6 +37,7 @@
 #include "oracle_barrier.hpp"
 #include "oracle_dividend.hpp"
 #include "oracle_lsm.hpp"
+#include "oracle_mlp.hpp"
 #include "oracle_market.hpp"
 #include "oracle_numerics.hpp"
 #include "oracle_payoff.hpp"
@@ -1651,6 +1652,16 @@ int oracle_cpqr_diagonals(const double* matrix, uint32_t rows, uint32_t cols, co
         for (uint32_t c = 0; c < cols; ++c) out[c] = c < f.diagonal_abs.size() ? f.diagonal_abs[c] : 0.0;
         return 0;
     } catch (const std::exception& e) { return fail(e, 1); }
+}
+
+// Continuation network: loss and gradient over n rows (x row-major n x 2).
+double oracle_mlp_gradients(const double* params, const double* x, const double* y, uint32_t n, int bf16,
+                            double* grad) {
+    return mlp_loss_gradient(params, x, y, n, bf16 != 0, grad);
+}
+
+void oracle_mlp_forward(const double* params, const double* x, uint32_t n, int bf16, double* out) {
+    for (uint32_t r = 0; r < n; ++r) out[r] = mlp_forward(params, x[2 * r], x[2 * r + 1], bf16 != 0);
 }
 
 // exp(-x/2) L_k(x), k = 0..degree.

@@ -1163,6 +1163,52 @@ pe_status pe_lsm_evaluate(const pe_plan* const* plans, uint32_t count, pe_backen
     });
 }
 
+pe_status pe_mlp_gradients(const float* params, const float* x, const float* y, uint32_t rows, float* grad_out,
+                           double* loss_out) {
+    if (!params || !x || !y || !grad_out || rows == 0) return fail(PE_ERR_INVALID_ARGUMENT, "null argument");
+    return guarded([&] {
+        ensure_init();
+        const uint32_t rows_wg = 256;
+        const uint32_t wgs = (rows + rows_wg - 1) / rows_wg;
+        const size_t np = kMlpParameterCount + 1;
+        DeviceBuffer dp, dx, dy, dpart;
+        dp.upload(std::vector<float>(params, params + kMlpParameterCount));
+        dx.upload(std::vector<float>(x, x + 2 * static_cast<size_t>(rows)));
+        dy.upload(std::vector<float>(y, y + rows));
+        dpart.reserve(static_cast<size_t>(wgs) * np * sizeof(float));
+        check(launch_mlp_gradient(dp.as<float>(), dx.as<float>(), dy.as<float>(), rows, rows_wg, wgs,
+                                  dpart.as<float>(), g_stream),
+              "MLP gradient launch");
+        std::vector<float> part(static_cast<size_t>(wgs) * np);
+        check(hipMemcpyAsync(part.data(), dpart.ptr, part.size() * sizeof(float), hipMemcpyDeviceToHost, g_stream),
+              "partials copy");
+        check(hipStreamSynchronize(g_stream), "hipStreamSynchronize");
+        for (size_t k = 0; k < np; ++k) {
+            double sum = 0.0;
+            for (uint32_t w = 0; w < wgs; ++w) sum += part[static_cast<size_t>(w) * np + k];
+            if (k < kMlpParameterCount) grad_out[k] = static_cast<float>(sum);
+            else if (loss_out) *loss_out = sum;
+        }
+    });
+}
+
+pe_status pe_mlp_forward(const float* params, const float* x, uint32_t rows, float* out) {
+    if (!params || !x || !out || rows == 0) return fail(PE_ERR_INVALID_ARGUMENT, "null argument");
+    return guarded([&] {
+        ensure_init();
+        DeviceBuffer dp, dx, dout;
+        dp.upload(std::vector<float>(params, params + kMlpParameterCount));
+        dx.upload(std::vector<float>(x, x + 2 * static_cast<size_t>(rows)));
+        dout.reserve(static_cast<size_t>(rows) * sizeof(float));
+        check(launch_mlp_forward(dp.as<float>(), dx.as<float>(), rows, dout.as<float>(), g_stream),
+              "MLP forward launch");
+        check(hipMemcpyAsync(out, dout.ptr, static_cast<size_t>(rows) * sizeof(float), hipMemcpyDeviceToHost,
+                             g_stream),
+              "output copy");
+        check(hipStreamSynchronize(g_stream), "hipStreamSynchronize");
+    });
+}
+
 pe_status pe_last_kernel_stats(double* avg_ms, uint32_t* launches) {
     if (avg_ms) *avg_ms = g_kernel_ms;
     if (launches) *launches = g_kernel_launches;

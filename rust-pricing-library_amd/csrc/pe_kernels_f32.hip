@@ -1,9 +1,12 @@
 // Mixed-precision kernels: FP32 transcendentals, inverse normal and variance
 // lookups; FP64 path accumulators and statistics.
+#include <algorithm>
+
 #include "pe_kernels.inc"
 #include "pe_bs.inc"
 #include "pe_rough.inc"
 #include "pe_lsm.inc"
+#include "pe_mlp.inc"
 #include "pe_plan.hpp"
 
 namespace pe {
@@ -20,6 +23,28 @@ hipError_t launch_rough_f32(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 
 hipError_t launch_lsm_f32(const LsmArgs& train, uint32_t train_items, const LsmArgs& value, uint32_t value_items,
                           uint32_t basis, hipStream_t s, hipEvent_t between) {
     return launch_lsm<float>(train, train_items, value, value_items, basis, s, between);
+}
+
+hipError_t launch_mlp_gradient(const float* params, const float* x, const float* y, uint32_t rows, uint32_t rows_wg,
+                               uint32_t wgs, float* partials, hipStream_t s) {
+    const size_t lds = mlp_lds_bytes();
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&mlp_gradient_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(mlp_gradient_kernel, dim3(wgs), dim3(kMlpThreads), lds, s, params, x, y, rows, rows_wg,
+                       partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_forward(const float* params, const float* x, uint32_t rows, float* out, hipStream_t s) {
+    const size_t lds = mlp_lds_bytes();
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&mlp_forward_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+    const uint32_t tiles = (rows + 15) / 16;
+    const uint32_t wgs = std::max<uint32_t>(1, std::min<uint32_t>(1024, (tiles + kMlpWaves - 1) / kMlpWaves));
+    hipLaunchKernelGGL(mlp_forward_kernel, dim3(wgs), dim3(kMlpThreads), lds, s, params, x, rows, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_lv_f32(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s) {

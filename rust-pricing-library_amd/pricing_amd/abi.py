@@ -289,6 +289,9 @@ PE_SYMBOLS = {
     ),
     "pe_lsm_evaluate": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.c_int, C.POINTER(pe_result),
                                   C.POINTER(pe_lsm_report)]),
+    "pe_mlp_gradients": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint32,
+                                   C.POINTER(C.c_float), _DP]),
+    "pe_mlp_forward": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint32, C.POINTER(C.c_float)]),
     "pe_last_kernel_stats": (C.c_int, [_DP, C.POINTER(C.c_uint32)]),
 }
 

@@ -75,6 +75,9 @@ def oracle():
     lib.oracle_cholesky_fit.argtypes = [D, C.c_uint32, C.c_uint32, D, C.c_double, C.c_double, D]
     lib.oracle_cpqr_diagonals.argtypes = [D, C.c_uint32, C.c_uint32, D, C.c_double, C.c_double, D]
     lib.oracle_weighted_laguerre.argtypes = [C.c_double, C.c_uint32, D]
+    lib.oracle_mlp_gradients.argtypes = [D, D, D, C.c_uint32, C.c_int, D]
+    lib.oracle_mlp_gradients.restype = C.c_double
+    lib.oracle_mlp_forward.argtypes = [D, D, C.c_uint32, C.c_int, D]
     lib.oracle_lifted_from_rough.argtypes = [C.c_double, C.c_uint32, C.c_double, D, D]
     lib.oracle_lifted_path.argtypes = [C.POINTER(abi.pe_lifted_heston), D, C.c_uint32, C.c_double, D, D, D]
     lib.oracle_rough_evaluate.argtypes = [C.POINTER(C.POINTER(abi.pe_request)), C.c_uint32,
