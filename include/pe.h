@@ -316,6 +316,12 @@ typedef struct pe_lsm_decision {
     int32_t pad;
     double rank_threshold;
     double residual_sum_squares;
+    /* MLP policy: the exercise region in moneyness S/K at this date, for
+     * in-the-money paths: inside iff region_left xor (number of cuts <= S/K
+     * is odd) */
+    int32_t region_left;
+    uint32_t region_cut_count;
+    float region_cuts[64];
 } pe_lsm_decision;
 
 /* Early-exercise diagnostics of one plan (EarlyExerciseDiagnostics,

@@ -8,8 +8,10 @@
 // Parameter vector (float32 on the device, FP64 here):
 //   W1[64][2] | b1[64] | W2[64][64] (row = output unit) | b2[64] | w3[64] | b3
 // With `bf16` set, the quantities the device feeds to its BF16 matrix cores
-// are rounded to bfloat16 (round to nearest even): W2, the hidden activations
-// h1 and the layer-2 pre-activation gradients; everything else stays FP64.
+// are rounded to bfloat16 (round to nearest even): W2, both layers' hidden
+// activations, both layers' pre-activation gradients, the inputs and the
+// output gradient where they enter a contraction over rows; everything else
+// stays FP64.
 #pragma once
 
 #include <cmath>
@@ -78,13 +80,14 @@ inline double mlp_loss_gradient(const double* p, const double* x, const double* 
         loss += e * e * inv;
         const double go = 2.0 * e * inv;
         g[kMlpB3] += go;
+        const double gob = bf16 ? bf16_round(go) : go;
         double gz[kMlpH];
         for (int o = 0; o < kMlpH; ++o) {
-            g[kMlpW3 + o] += go * h2[o];
+            g[kMlpW3 + o] += gob * (bf16 ? bf16_round(h2[o]) : h2[o]);
             const double s = sigmoid(z2[o]);
             gz[o] = go * p[kMlpW3 + o] * s * (1.0 + z2[o] * (1.0 - s));
-            g[kMlpB2 + o] += gz[o];
             if (bf16) gz[o] = bf16_round(gz[o]);
+            g[kMlpB2 + o] += gz[o];
         }
         for (int o = 0; o < kMlpH; ++o)
             for (int f = 0; f < kMlpH; ++f) g[kMlpW2 + kMlpH * o + f] += gz[o] * (bf16 ? bf16_round(h1[f]) : h1[f]);
@@ -93,10 +96,11 @@ inline double mlp_loss_gradient(const double* p, const double* x, const double* 
             for (int o = 0; o < kMlpH; ++o)
                 gh += (bf16 ? bf16_round(p[kMlpW2 + kMlpH * o + f]) : p[kMlpW2 + kMlpH * o + f]) * gz[o];
             const double s = sigmoid(a1[f]);
-            const double ga = gh * s * (1.0 + a1[f] * (1.0 - s));
+            double ga = gh * s * (1.0 + a1[f] * (1.0 - s));
+            if (bf16) ga = bf16_round(ga);
             g[kMlpB1 + f] += ga;
-            g[kMlpW1 + 2 * f] += ga * x1;
-            g[kMlpW1 + 2 * f + 1] += ga * x2;
+            g[kMlpW1 + 2 * f] += ga * (bf16 ? bf16_round(x1) : x1);
+            g[kMlpW1 + 2 * f + 1] += ga * (bf16 ? bf16_round(x2) : x2);
         }
     }
     std::memcpy(grad, g.data(), sizeof(double) * kMlpParams);

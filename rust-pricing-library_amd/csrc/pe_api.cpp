@@ -813,6 +813,13 @@ struct LsmLaunch {
     std::vector<unsigned long long> host_counts;
     std::vector<double> host_in_sample;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    // network policy
+    bool mlp = false;
+    uint32_t mlp_wgs = 0;
+    MlpLsmArgs mlp_args{};
+    std::vector<float> mlp_init;
+    std::vector<LsmCuts> host_cuts;
+    DeviceBuffer weights, adam_m, adam_v, mlp_partials, mlp_rows, mlp_stats, cuts, mlp_err;
     ~LsmLaunch() {
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -822,6 +829,81 @@ struct LsmLaunch {
 void LsmLaunchDeleter::operator()(LsmLaunch* l) const { delete l; }
 
 namespace {
+
+// The network policy shares one network per exercise date across the
+// launch: every plan must have the same exercise days, side, training engine
+// shape and network settings.  He-uniform initial weights from SplitMix64 of
+// the network seed; Adam state starts at zero for every evaluation.
+void prepare_mlp(LsmLaunch& L, std::vector<HostPlan*>& plans) {
+    const HostPlan& p0 = *plans[0];
+    const pe_lsm& l = p0.req.lsm;
+    for (const HostPlan* q : plans) {
+        const pe_lsm& o = q->req.lsm;
+        if (q->obs_days != p0.obs_days || q->req.product.side != p0.req.product.side ||
+            q->lsm_trade.train_paths != p0.lsm_trade.train_paths ||
+            q->lsm_trade.train.antithetic != p0.lsm_trade.train.antithetic || o.mlp_hidden != l.mlp_hidden ||
+            o.mlp_epochs != l.mlp_epochs || o.mlp_batch != l.mlp_batch || o.mlp_learning_rate != l.mlp_learning_rate ||
+            o.mlp_seed != l.mlp_seed)
+            throw PlanError(PE_ERR_INVALID_ARGUMENT,
+                            "plans sharing the MLP policy need one exercise schedule, side, training shape and network");
+    }
+    if (l.mlp_hidden != 64) throw PlanError(PE_ERR_UNSUPPORTED, "MLP hidden width must be 64 on the device");
+    if (l.mlp_epochs == 0 || !(l.mlp_learning_rate > 0.0)) throw PlanError(PE_ERR_INVALID_ARGUMENT, "MLP epochs / rate");
+    const uint32_t B = l.mlp_batch;
+    // 16 workgroups of eight waves per step (the grid barrier costs grow with
+    // the workgroup count); each wave takes 64 rows of an 8192-row batch
+    const uint32_t G = std::min<uint32_t>(16, B / 512);
+    if (G == 0 || (B / G) % 128 != 0 || (B / G) * G != B || B / G > 2048)
+        throw PlanError(PE_ERR_INVALID_ARGUMENT, "MLP batch must be a multiple of 512, at most 32768");
+    L.mlp = true;
+    L.mlp_wgs = G;
+    // He-uniform initial weights
+    L.mlp_init.assign(kMlpParameterCount, 0.0f);
+    uint64_t state = l.mlp_seed;
+    auto uniform = [&]() {
+        state += 0x9E3779B97F4A7C15ull;
+        uint64_t z = state;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return 2.0 * (static_cast<double>(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+    };
+    for (int k = 0; k < 128; ++k) L.mlp_init[k] = static_cast<float>(uniform() * std::sqrt(6.0 / 2.0));
+    for (int k = 192; k < 4288; ++k) L.mlp_init[k] = static_cast<float>(uniform() * std::sqrt(6.0 / 64.0));
+    for (int k = 4352; k < 4416; ++k) L.mlp_init[k] = static_cast<float>(uniform() * std::sqrt(3.0 / 64.0));
+    const uint64_t pool = static_cast<uint64_t>(plans.size()) * p0.lsm_trade.train_paths;
+    L.weights.reserve(kMlpParameterCount * sizeof(float));
+    L.adam_m.reserve(kMlpParameterCount * sizeof(float));
+    L.adam_v.reserve(kMlpParameterCount * sizeof(float));
+    L.mlp_partials.reserve(static_cast<size_t>(G) * (kMlpParameterCount + 1) * sizeof(float));
+    L.mlp_rows.reserve(static_cast<size_t>(pool) * 3 * sizeof(float) + 16);
+    L.mlp_stats.reserve(static_cast<size_t>(G) * 8 * sizeof(double));
+    L.cuts.reserve(std::max<size_t>(1, L.n_decisions) * sizeof(LsmCuts));
+    L.mlp_err.reserve(sizeof(unsigned int));
+    L.host_cuts.resize(L.n_decisions);
+    MlpLsmArgs& m = L.mlp_args;
+    m.weights = L.weights.as<float>();
+    m.adam_m = L.adam_m.as<float>();
+    m.adam_v = L.adam_v.as<float>();
+    m.partials = L.mlp_partials.as<float>();
+    m.rows = L.mlp_rows.as<float>();
+    m.stats = L.mlp_stats.as<double>();
+    m.cuts = L.cuts.as<LsmCuts>();
+    m.error = L.mlp_err.as<unsigned int>();
+    m.n_trades = static_cast<uint32_t>(plans.size());
+    m.n_dates = p0.lsm_trade.n_dates;
+    m.paths_per_trade = static_cast<uint32_t>(p0.lsm_trade.train_paths);
+    m.batch = B;
+    m.epochs = l.mlp_epochs;
+    m.rows_per_wg = B / G;
+    m.lr = static_cast<float>(l.mlp_learning_rate);
+    m.beta1 = 0.9f;
+    m.beta2 = 0.999f;
+    m.eps = 1e-8f;
+    m.call = p0.req.product.side == PE_CALL ? 1u : 0u;
+    m.grid_lo = m.call ? 1.0f : 0.25f;
+    m.grid_hi = m.call ? 4.0f : 1.0f;
+}
 
 LsmLaunch& prepare_lsm(std::vector<HostPlan*>& plans, pe_backend backend) {
     HostPlan& p0 = *plans[0];
@@ -841,8 +923,7 @@ LsmLaunch& prepare_lsm(std::vector<HostPlan*>& plans, pe_backend backend) {
     for (size_t ti = 0; ti < plans.size(); ++ti) {
         HostPlan& p = *plans[ti];
         if (!p.american) throw PlanError(PE_ERR_INVALID_ARGUMENT, "not an early-exercise plan");
-        if (p.req.lsm.method == PE_LSM_MLP) throw PlanError(PE_ERR_UNSUPPORTED, "MLP policy in a polynomial batch");
-        if (p.lsm_trade.basis != L->basis) throw PlanError(PE_ERR_INVALID_ARGUMENT, "one LSM basis per launch");
+        if (p.lsm_trade.basis != L->basis) throw PlanError(PE_ERR_INVALID_ARGUMENT, "one LSM method per launch");
         LsmTrade t = p.lsm_trade;
         const uint32_t n = t.n_dates;
         t.date_offset = static_cast<uint32_t>(dates.size());
@@ -912,6 +993,7 @@ LsmLaunch& prepare_lsm(std::vector<HostPlan*>& plans, pe_backend backend) {
     L->host_counts.resize(L->n_counts);
     L->host_in_sample.resize(plans.size());
     for (hipEvent_t& e : L->ev) check(hipEventCreate(&e), "hipEventCreate");
+    if (L->basis == kLsmMlp) prepare_mlp(*L, plans);
     p0.lsm_launch.reset(L.release());
     return *p0.lsm_launch;
 }
@@ -938,11 +1020,30 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
     train.n_items = static_cast<uint32_t>(L.train_items.size());
     value.items = L.d_value_items.as<LsmItem>();
     value.n_items = static_cast<uint32_t>(L.value_items.size());
+    if (L.mlp) {
+        train.cuts = value.cuts = L.cuts.as<LsmCuts>();
+        L.mlp_args.lsm = train;
+        check(hipMemcpyAsync(L.weights.ptr, L.mlp_init.data(), kMlpParameterCount * sizeof(float),
+                             hipMemcpyHostToDevice, g_stream),
+              "MLP weights");
+        check(hipMemsetAsync(L.adam_m.ptr, 0, kMlpParameterCount * sizeof(float), g_stream), "Adam state");
+        check(hipMemsetAsync(L.adam_v.ptr, 0, kMlpParameterCount * sizeof(float), g_stream), "Adam state");
+        check(hipMemsetAsync(L.mlp_err.ptr, 0, sizeof(unsigned int), g_stream), "MLP error word");
+        check(hipMemsetAsync(L.cuts.ptr, 0, std::max<size_t>(1, L.n_decisions) * sizeof(LsmCuts), g_stream), "cuts");
+    }
     check(hipEventRecord(L.ev[0], g_stream), "hipEventRecord");
-    check(backend == PE_BACKEND_GPU_FP64
-              ? launch_lsm_f64(train, train.n_items, value, value.n_items, L.basis, g_stream, L.ev[1])
-              : launch_lsm_f32(train, train.n_items, value, value.n_items, L.basis, g_stream, L.ev[1]),
-          "LSM kernel launch");
+    if (L.mlp)
+        check(backend == PE_BACKEND_GPU_FP64
+                  ? launch_lsm_mlp_f64(train, train.n_items, L.mlp_args, L.mlp_wgs, value, value.n_items, g_stream,
+                                       L.ev[1])
+                  : launch_lsm_mlp_f32(train, train.n_items, L.mlp_args, L.mlp_wgs, value, value.n_items, g_stream,
+                                       L.ev[1]),
+              "MLP LSM launch");
+    else
+        check(backend == PE_BACKEND_GPU_FP64
+                  ? launch_lsm_f64(train, train.n_items, value, value.n_items, L.basis, g_stream, L.ev[1])
+                  : launch_lsm_f32(train, train.n_items, value, value.n_items, L.basis, g_stream, L.ev[1]),
+              "LSM kernel launch");
     check(hipEventRecord(L.ev[2], g_stream), "hipEventRecord");
     check(hipEventSynchronize(L.ev[2]), "hipEventSynchronize");
     float train_ms = 0.0f, value_ms = 0.0f;
@@ -966,6 +1067,14 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
         throw PlanError(PE_ERR_NONFINITE_STATE, "trade " + std::to_string(trade) + ", sampling unit " +
                                                     std::to_string(unit) + ": non-finite early-exercise path");
     }
+    if (L.mlp) {
+        unsigned int merr = 0;
+        check(hipMemcpy(&merr, L.mlp_err.ptr, sizeof(merr), hipMemcpyDeviceToHost), "MLP error copy");
+        if (merr) throw PlanError(PE_ERR_SIZE_LIMIT, "an MLP exercise region needs more moneyness cuts than the grid allows");
+        if (L.n_decisions)
+            check(hipMemcpy(L.host_cuts.data(), L.cuts.ptr, L.n_decisions * sizeof(LsmCuts), hipMemcpyDeviceToHost),
+                  "cuts copy");
+    }
     for (size_t ti = 0; ti < plans.size(); ++ti) {
         const HostPlan& p = *plans[ti];
         const LsmTrade& t = L.trades[ti];
@@ -988,6 +1097,13 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
             const LsmDecision& s = L.host_decisions[t.decision_offset + d];
             pe_lsm_decision& o = r.decisions[d];
             std::memset(&o, 0, sizeof(o));
+            if (L.mlp) {
+                const LsmCuts& c = L.host_cuts[t.decision_offset + d];
+                o.region_left = static_cast<int32_t>(c.left);
+                o.region_cut_count = c.count;
+                for (uint32_t k = 0; k < c.count && k < 64; ++k) o.region_cuts[k] = c.cut[k];
+                continue;
+            }
             o.continue_all = s.continue_all ? 1 : 0;
             o.itm_rows = s.itm_rows;
             if (s.continue_all) continue;

@@ -2,11 +2,14 @@
 // lookups; FP64 path accumulators and statistics.
 #include <algorithm>
 
+#define PE_MLP_VERIFY_KERNELS 1
+
 #include "pe_kernels.inc"
 #include "pe_bs.inc"
 #include "pe_rough.inc"
 #include "pe_lsm.inc"
 #include "pe_mlp.inc"
+#include "pe_mlp_lsm.inc"
 #include "pe_plan.hpp"
 
 namespace pe {
@@ -45,6 +48,11 @@ hipError_t launch_mlp_forward(const float* params, const float* x, uint32_t rows
     const uint32_t wgs = std::max<uint32_t>(1, std::min<uint32_t>(1024, (tiles + kMlpWaves - 1) / kMlpWaves));
     hipLaunchKernelGGL(mlp_forward_kernel, dim3(wgs), dim3(kMlpThreads), lds, s, params, x, rows, out);
     return hipGetLastError();
+}
+
+hipError_t launch_lsm_mlp_f32(const LsmArgs& train, uint32_t train_items, const MlpLsmArgs& m, uint32_t wgs,
+                              const LsmArgs& value, uint32_t value_items, hipStream_t s, hipEvent_t between) {
+    return launch_lsm_mlp<float>(train, train_items, m, wgs, value, value_items, s, between);
 }
 
 hipError_t launch_lv_f32(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s) {

@@ -4,6 +4,8 @@
 #include "pe_bs.inc"
 #include "pe_rough.inc"
 #include "pe_lsm.inc"
+#include "pe_mlp.inc"
+#include "pe_mlp_lsm.inc"
 #include "pe_plan.hpp"
 
 namespace pe {
@@ -20,6 +22,11 @@ hipError_t launch_rough_f64(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 
 hipError_t launch_lsm_f64(const LsmArgs& train, uint32_t train_items, const LsmArgs& value, uint32_t value_items,
                           uint32_t basis, hipStream_t s, hipEvent_t between) {
     return launch_lsm<double>(train, train_items, value, value_items, basis, s, between);
+}
+
+hipError_t launch_lsm_mlp_f64(const LsmArgs& train, uint32_t train_items, const MlpLsmArgs& m, uint32_t wgs,
+                              const LsmArgs& value, uint32_t value_items, hipStream_t s, hipEvent_t between) {
+    return launch_lsm_mlp<double>(train, train_items, m, wgs, value, value_items, s, between);
 }
 
 hipError_t launch_lv_f64(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s) {

@@ -580,7 +580,7 @@ void compile_lsm_trade(HostPlan& p) {
     t.call = p.req.product.side == PE_CALL ? 1u : 0u;
     t.zero_variance = p.total_variance == 0.0 ? 1u : 0u;
     t.single_terminal = n == 1 ? 1u : 0u;
-    t.basis = l.method == PE_LSM_POLY_LAGUERRE || l.method == PE_LSM_MLP ? kLsmLaguerre : kLsmMonomial;
+    t.basis = l.method == PE_LSM_MLP ? kLsmMlp : l.method == PE_LSM_POLY_LAGUERRE ? kLsmLaguerre : kLsmMonomial;
     t.columns = l.max_degree + 1;
     t.strike = p.req.product.strike;
     t.notional = p.req.product.notional;

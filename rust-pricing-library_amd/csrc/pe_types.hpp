@@ -239,6 +239,18 @@ constexpr int kLsmMaxDates = 1024;    // exercise dates per trade
 enum LsmBasisKind : uint32_t {
     kLsmMonomial = 0,   // standardized monomials, CPQR pivot/rank rules, all-path decisions
     kLsmLaguerre = 1,   // exp(-x/2) L_k(x), x = S / K, in-the-money decisions
+    kLsmMlp = 2,        // continuation network; decisions from its exercise region in moneyness
+};
+
+constexpr int kLsmMaxCuts = 64;   // the region search has 64 grid points: at most 63 sign changes
+
+// Exercise region of the network policy at one (trade, date): in-the-money
+// moneyness m = S / K is in the region iff `left` xor (number of cuts <= m
+// is odd).  The cuts are the sign changes of immediate - continuation,
+// located by bisection on the network itself.
+struct LsmCuts {
+    uint32_t count, left;
+    float cut[kLsmMaxCuts];
 };
 
 // One exercise date of a trade: exact BS/Black-76 step and the escrowed
@@ -304,7 +316,29 @@ struct LsmArgs {
     double* in_sample;                // per trade: mean of the training cash flows
     void* partials;                   // BlockStat per valuation item
     unsigned long long* error;
+    const LsmCuts* cuts;              // network policy: [trade][non-terminal date]
     uint32_t n_items, n_trades;
+};
+
+// The learned policy's persistent training launch: one network per exercise
+// date shared by every trade of the launch (inputs moneyness and volatility,
+// output the continuation value per unit strike), trained on the pooled
+// in-the-money training rows.
+struct MlpLsmArgs {
+    LsmArgs lsm;
+    float* weights;            // FP32 master parameters
+    float* adam_m;
+    float* adam_v;
+    float* partials;           // [workgroup][parameters + 1]
+    float* rows;               // pooled training rows: (u1, u2, target) per row
+    double* stats;             // [workgroup][8]: count, sums of m, m^2, vol, vol^2, y, y^2
+    LsmCuts* cuts;             // [trade][non-terminal date]
+    unsigned int* error;       // bit 0: an exercise region needed more than kLsmMaxCuts cuts
+    uint32_t n_trades, n_dates, paths_per_trade;
+    uint32_t batch, epochs, rows_per_wg;
+    float lr, beta1, beta2, eps;
+    float grid_lo, grid_hi;    // moneyness range of the exercise-region search
+    uint32_t call, pad;
 };
 
 // Kernel arguments of the local-vol Log-Euler kernel.

@@ -255,6 +255,9 @@ class pe_lsm_decision(C.Structure):
         ("pad", C.c_int32),
         ("rank_threshold", C.c_double),
         ("residual_sum_squares", C.c_double),
+        ("region_left", C.c_int32),
+        ("region_cut_count", C.c_uint32),
+        ("region_cuts", C.c_float * 64),
     ]
 
 

@@ -219,7 +219,8 @@ AMERICAN_RATE = 0.03
 
 
 def american_batch(count=256, seed=0x5EED_1A5E, train_log2=18, value_log2=18, dates=50,
-                   method=abi.PE_LSM_POLY_LAGUERRE, antithetic=False, dividends=False):
+                   method=abi.PE_LSM_POLY_LAGUERRE, antithetic=False, dividends=False, mlp_epochs=4,
+                   mlp_batch=8192):
     """American puts priced by LSM: S0 = 100, K ~ U[80, 120], sigma ~ U[0.2,
     0.4], r = 3%, q = 0, T = 1 year, `dates` exercise dates at round(365 k /
     dates) days; per trade 2^train_log2 training paths (pseudo-MC, domain
@@ -242,7 +243,8 @@ def american_batch(count=256, seed=0x5EED_1A5E, train_log2=18, value_log2=18, da
         req.set_product(abi.PE_PRODUCT_AMERICAN, abi.PE_PUT, days[-1], strike, dates=days)
         req.set_pseudo(seed_v, 1 << value_log2, antithetic=antithetic)
         req.set_risk()
-        req.set_lsm((abi.PE_ENGINE_PSEUDO, seed_t, 1 << train_log2, 0), antithetic=antithetic, method=method)
+        req.set_lsm((abi.PE_ENGINE_PSEUDO, seed_t, 1 << train_log2, 0), antithetic=antithetic, method=method,
+                    mlp_epochs=mlp_epochs, mlp_batch=mlp_batch)
         req.trade_id = tid
         out.append(req)
     return out
