@@ -14,7 +14,9 @@ Workload (one "step" = one evaluation of the whole job):
 A second block ("batch_bs") times the batched BS/Black-76 workload: 4096
 uneven trades with affine dividends and Delta/Vega, mixed vs FP64 device modes
 and the CPU oracle on a sample of the trades.  A third ("rough_smile") times
-the lifted Heston 21 x 8 European smile on one 2^20-path RQMC set.
+the lifted Heston 21 x 8 European smile on one 2^20-path RQMC set.  A fourth
+("lsm_american") prices 256 American puts by LSM with the weighted-Laguerre
+policy and with the learned (MLP) policy.
 
 Prints one JSON line.  --gpus above 1 is refused: this build runs one GPU.
 """
@@ -240,6 +242,89 @@ def rough_block(steps, cpu_seconds, no_cpu, no_fp64):
     return block
 
 
+def lsm_block(cpu_seconds, no_cpu, no_fp64):
+    """Config 4: 256 American puts by LSM, 50 exercise dates, 2^18 training +
+    2^18 independent valuation paths per trade.  Plain policy: weighted
+    Laguerre basis of degree 3 with FP64 Cholesky, fitted per trade and date.
+    Learned policy: one 2-64-64-1 SiLU network per date shared by the 256
+    trades (moneyness and volatility inputs), BF16 matrix cores, Adam lr 1e-3,
+    4 epochs, batch 8192, trained on 2^8 paths per trade (2^16 pooled)."""
+    import numpy as np
+
+    def run(method, train_log2, backend, reps=2):
+        reqs = workloads.american_batch(256, train_log2=train_log2, value_log2=18, dates=50, method=method)
+        plans = [engine.PricingPlan.compile(r) for r in reqs]
+        engine.evaluate_lsm(plans, backend)
+        walls = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            res, rep = engine.evaluate_lsm(plans, backend)
+            walls.append(time.perf_counter() - t0)
+        return reqs, res, rep, min(walls)
+
+    reqs, poly, prep, pwall = run(abi.PE_LSM_POLY_LAGUERRE, 18, "mixed")
+    # nominal path-steps: every training and valuation path over every date
+    train_steps = sum(r.c.lsm.training_engine.units * r.c.product.date_count for r in reqs)
+    value_steps = sum(r.c.engine.units * r.c.product.date_count for r in reqs)
+    pms = prep[0].training_ms + prep[0].valuation_ms
+    block = {
+        "workload": "256 American puts, S0=100, K~U[80,120], sigma~U[0.2,0.4], r=3%, T=1y, 50 exercise dates, "
+                    "2^18 training + 2^18 valuation pseudo paths per trade",
+        "poly_laguerre": {
+            "device_ms": pms, "training_ms": prep[0].training_ms, "valuation_ms": prep[0].valuation_ms,
+            "wall_ms": 1e3 * pwall,
+            "path_steps": train_steps + value_steps,
+            "value": (train_steps + value_steps) / (pms * 1e-3), "unit": "path-steps/s (training + valuation)",
+            "mean_price": float(np.mean([r.estimates[0].value for r in poly])),
+            "mean_se": float(np.mean([r.estimates[0].standard_error for r in poly])),
+        },
+    }
+    _, nn, nrep, nwall = run(abi.PE_LSM_MLP, 8, "mixed")
+    nms = nrep[0].training_ms + nrep[0].valuation_ms
+    se = np.array([r.estimates[0].standard_error for r in poly])
+    d = np.array([a.estimates[0].value - b.estimates[0].value for a, b in zip(nn, poly)])
+    block["mlp"] = {
+        "device_ms": nms, "training_ms": nrep[0].training_ms, "valuation_ms": nrep[0].valuation_ms,
+        "wall_ms": 1e3 * nwall,
+        "training_paths_per_trade": 1 << 8,
+        "mean_price": float(np.mean([r.estimates[0].value for r in nn])),
+        "mean_minus_poly_in_se": float(np.mean(d / se)),
+        "max_abs_minus_poly_in_se": float(np.max(np.abs(d) / se)),
+        "time_over_poly": nms / pms,
+    }
+    if not no_fp64:
+        _, f64, frep, fwall = run(abi.PE_LSM_POLY_LAGUERRE, 18, "fp64", reps=1)
+        fms = frep[0].training_ms + frep[0].valuation_ms
+        block["poly_laguerre"]["fp64_device_ms"] = fms
+        block["poly_laguerre"]["mixed_over_fp64"] = fms / pms
+        block["poly_laguerre"]["max_mixed_minus_fp64_in_se"] = max(
+            abs(a.estimates[0].value - b.estimates[0].value) / b.estimates[0].standard_error for a, b in zip(poly, f64))
+    if not no_cpu:
+        from concurrent.futures import ThreadPoolExecutor
+
+        from _support import oracle, oracle_lsm
+
+        cores = int(oracle().oracle_hardware_threads())
+        log2 = 10
+        while True:
+            sample = workloads.american_batch(cores, train_log2=log2, value_log2=log2, dates=50)
+            t0 = time.perf_counter()
+            with ThreadPoolExecutor(max_workers=cores) as pool:
+                out = list(pool.map(lambda r: oracle_lsm(r, 4096, 1)[0], sample))
+            dt = time.perf_counter() - t0
+            if dt >= 0.25 * cpu_seconds or log2 >= 18:
+                break
+            log2 += max(1, min(4, int(np.log2(max(2.0, cpu_seconds / max(dt, 1e-3))))))
+            log2 = min(log2, 18)
+        steps = len(out) * 2 * (1 << log2) * 50
+        block["cpu_baseline"] = {
+            "value": steps / dt, "unit": "path-steps/s (training + valuation)", "cores": cores, "kind": "port",
+            "sample": f"{cores} trades of the same workload at 2^{log2} training + 2^{log2} valuation paths, "
+                      f"weighted-Laguerre LSM in the FP64 oracle, one trade per core, {dt:.2f} s wall"}
+        block["speedup_vs_cpu_all_cores"] = block["poly_laguerre"]["value"] / block["cpu_baseline"]["value"]
+    return block
+
+
 def copy_probe(nbytes):
     """A plain device-to-device copy of known size: the HBM counters it reads
     (FETCH_SIZE, WRITE_SIZE) calibrate the per-dispatch traffic figures."""
@@ -269,6 +354,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 4096-trade BS/Black-76 batch block")
     ap.add_argument("--no-rough", action="store_true", help="skip the lifted Heston smile block")
+    ap.add_argument("--no-lsm", action="store_true", help="skip the 256-trade American LSM block")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -352,6 +438,8 @@ def main():
         out["batch_bs"] = batch_block(3, args.cpu_seconds, args.no_cpu, args.no_fp64)
     if not args.no_rough:
         out["rough_smile"] = rough_block(5, args.cpu_seconds, args.no_cpu, args.no_fp64)
+    if not args.no_lsm:
+        out["lsm_american"] = lsm_block(args.cpu_seconds, args.no_cpu, args.no_fp64)
     print(json.dumps(out))
     return 0
 
