@@ -36,6 +36,7 @@ from pricing_amd import abi, engine, request, workloads  # noqa: E402
 METRIC = "path-steps/s at matched RQMC std-error vs FP64 oracle; NN-LSM vs poly-LSM time"
 FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
 FP64_PEAK_TFLOPS = 78.6    # MI355X published FP64 vector peak
+HBM_PEAK_GBS = 8000.0      # MI355X published HBM3E bandwidth
 # Frozen work model (flops per path-step of the local-vol path-dependent set):
 # normal draw 34 + bridge 7 + LV Log-Euler step 24 + barrier monitor 4 +
 # Asian accumulate 2 + observation 2.
@@ -52,6 +53,17 @@ def measured_traffic():
             return json.load(f)["traffic_bytes"]
     except (OSError, KeyError, ValueError):
         return None
+
+
+def lsm_traffic():
+    """HBM bytes per launch of the stored-path LSM kernels from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes, scaled by the same counters on
+    a copy of known size (profiles/lsm_kernel_traffic.json)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "lsm_kernel_traffic.json")) as f:
+            return {k: v["traffic_bytes"] for k, v in json.load(f)["kernels"].items()}
+    except (OSError, KeyError, ValueError):
+        return {}
 
 
 def build_plans(points, scrambles):
@@ -242,7 +254,7 @@ def rough_block(steps, cpu_seconds, no_cpu, no_fp64):
     return block
 
 
-def lsm_block(cpu_seconds, no_cpu, no_fp64):
+def lsm_block(cpu_seconds, no_cpu, no_fp64, no_mlp=False):
     """Config 4: 256 American puts by LSM, 50 exercise dates, 2^18 training +
     2^18 independent valuation paths per trade.  Plain policy: weighted
     Laguerre basis of degree 3 with FP64 Cholesky, fitted per trade and date.
@@ -267,9 +279,35 @@ def lsm_block(cpu_seconds, no_cpu, no_fp64):
     train_steps = sum(r.c.lsm.training_engine.units * r.c.product.date_count for r in reqs)
     value_steps = sum(r.c.engine.units * r.c.product.date_count for r in reqs)
     pms = prep[0].training_ms + prep[0].valuation_ms
+    # Rooflines of the two stored-path kernels (HBM-bound by design).  The
+    # path kernel writes the date-major FP32 training store once: 4 B per
+    # (trade, date, path).  The induction kernel's algorithmic traffic: the
+    # store read once (4 B per path-date) plus the FP32 cash flow read and
+    # written once per non-terminal date (8 B per path-date).
+    rep = prep[0]
+    paths = sum(r.c.lsm.training_engine.units for r in reqs)
+    dates = reqs[0].c.product.date_count
+    induction_ms = rep.training_ms - rep.path_ms
+    induction_bytes = paths * (4 * dates + 8 * (dates - 1))
+    store_bytes = paths * dates * 4
+    traffic = lsm_traffic()
     block = {
         "workload": "256 American puts, S0=100, K~U[80,120], sigma~U[0.2,0.4], r=3%, T=1y, 50 exercise dates, "
                     "2^18 training + 2^18 valuation pseudo paths per trade",
+        "roofline": {
+            "bound": "hbm", "kernel": "lsm_induction_kernel<float, 1u>", "avg_kernel_ms": induction_ms,
+            "algorithmic_bytes": induction_bytes,
+            "achieved": induction_bytes / (induction_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": induction_bytes / (induction_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "traffic": traffic.get("lsm_induction_kernel<float, 1u>"),
+        },
+        "path_store_roofline": {
+            "bound": "hbm", "kernel": "lsm_train_paths_kernel<float>", "avg_kernel_ms": rep.path_ms,
+            "algorithmic_bytes": store_bytes,
+            "achieved": store_bytes / (rep.path_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": store_bytes / (rep.path_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "traffic": traffic.get("lsm_train_paths_kernel<float>"),
+        },
         "poly_laguerre": {
             "device_ms": pms, "training_ms": prep[0].training_ms, "valuation_ms": prep[0].valuation_ms,
             "wall_ms": 1e3 * pwall,
@@ -278,19 +316,6 @@ def lsm_block(cpu_seconds, no_cpu, no_fp64):
             "mean_price": float(np.mean([r.estimates[0].value for r in poly])),
             "mean_se": float(np.mean([r.estimates[0].standard_error for r in poly])),
         },
-    }
-    _, nn, nrep, nwall = run(abi.PE_LSM_MLP, 8, "mixed")
-    nms = nrep[0].training_ms + nrep[0].valuation_ms
-    se = np.array([r.estimates[0].standard_error for r in poly])
-    d = np.array([a.estimates[0].value - b.estimates[0].value for a, b in zip(nn, poly)])
-    block["mlp"] = {
-        "device_ms": nms, "training_ms": nrep[0].training_ms, "valuation_ms": nrep[0].valuation_ms,
-        "wall_ms": 1e3 * nwall,
-        "training_paths_per_trade": 1 << 8,
-        "mean_price": float(np.mean([r.estimates[0].value for r in nn])),
-        "mean_minus_poly_in_se": float(np.mean(d / se)),
-        "max_abs_minus_poly_in_se": float(np.max(np.abs(d) / se)),
-        "time_over_poly": nms / pms,
     }
     if not no_fp64:
         _, f64, frep, fwall = run(abi.PE_LSM_POLY_LAGUERRE, 18, "fp64", reps=1)
@@ -322,6 +347,21 @@ def lsm_block(cpu_seconds, no_cpu, no_fp64):
             "sample": f"{cores} trades of the same workload at 2^{log2} training + 2^{log2} valuation paths, "
                       f"weighted-Laguerre LSM in the FP64 oracle, one trade per core, {dt:.2f} s wall"}
         block["speedup_vs_cpu_all_cores"] = block["poly_laguerre"]["value"] / block["cpu_baseline"]["value"]
+    if no_mlp:
+        return block
+    _, nn, nrep, nwall = run(abi.PE_LSM_MLP, 8, "mixed")
+    nms = nrep[0].training_ms + nrep[0].valuation_ms
+    se = np.array([r.estimates[0].standard_error for r in poly])
+    d = np.array([a.estimates[0].value - b.estimates[0].value for a, b in zip(nn, poly)])
+    block["mlp"] = {
+        "device_ms": nms, "training_ms": nrep[0].training_ms, "valuation_ms": nrep[0].valuation_ms,
+        "wall_ms": 1e3 * nwall,
+        "training_paths_per_trade": 1 << 8,
+        "mean_price": float(np.mean([r.estimates[0].value for r in nn])),
+        "mean_minus_poly_in_se": float(np.mean(d / se)),
+        "max_abs_minus_poly_in_se": float(np.max(np.abs(d) / se)),
+        "time_over_poly": nms / pms,
+    }
     return block
 
 
@@ -355,6 +395,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 4096-trade BS/Black-76 batch block")
     ap.add_argument("--no-rough", action="store_true", help="skip the lifted Heston smile block")
     ap.add_argument("--no-lsm", action="store_true", help="skip the 256-trade American LSM block")
+    ap.add_argument("--no-mlp", action="store_true",
+                    help="skip the learned-policy leg of the LSM block (its cooperative launch is not profiled)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -439,7 +481,7 @@ def main():
     if not args.no_rough:
         out["rough_smile"] = rough_block(5, args.cpu_seconds, args.no_cpu, args.no_fp64)
     if not args.no_lsm:
-        out["lsm_american"] = lsm_block(args.cpu_seconds, args.no_cpu, args.no_fp64)
+        out["lsm_american"] = lsm_block(args.cpu_seconds, args.no_cpu, args.no_fp64, args.no_mlp)
     print(json.dumps(out))
     return 0
 

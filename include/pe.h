@@ -336,6 +336,7 @@ typedef struct pe_lsm_report {
     uint32_t capacity;              /* entries of the caller arrays below */
     uint64_t* exercise_counts;      /* [capacity] or NULL */
     pe_lsm_decision* decisions;     /* [capacity] or NULL */
+    double path_ms;                 /* training-path generation, part of training_ms */
 } pe_lsm_report;
 
 /* Prices `count` American plans by least-squares Monte Carlo

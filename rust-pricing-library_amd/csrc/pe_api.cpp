@@ -812,7 +812,7 @@ struct LsmLaunch {
     std::vector<LsmDecision> host_decisions;
     std::vector<unsigned long long> host_counts;
     std::vector<double> host_in_sample;
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};   // start, training done, valuation done, paths done
     // network policy
     bool mlp = false;
     uint32_t mlp_wgs = 0;
@@ -1035,19 +1035,20 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
     if (L.mlp)
         check(backend == PE_BACKEND_GPU_FP64
                   ? launch_lsm_mlp_f64(train, train.n_items, L.mlp_args, L.mlp_wgs, value, value.n_items, g_stream,
-                                       L.ev[1])
+                                       L.ev[3], L.ev[1])
                   : launch_lsm_mlp_f32(train, train.n_items, L.mlp_args, L.mlp_wgs, value, value.n_items, g_stream,
-                                       L.ev[1]),
+                                       L.ev[3], L.ev[1]),
               "MLP LSM launch");
     else
         check(backend == PE_BACKEND_GPU_FP64
-                  ? launch_lsm_f64(train, train.n_items, value, value.n_items, L.basis, g_stream, L.ev[1])
-                  : launch_lsm_f32(train, train.n_items, value, value.n_items, L.basis, g_stream, L.ev[1]),
+                  ? launch_lsm_f64(train, train.n_items, value, value.n_items, L.basis, g_stream, L.ev[3], L.ev[1])
+                  : launch_lsm_f32(train, train.n_items, value, value.n_items, L.basis, g_stream, L.ev[3], L.ev[1]),
               "LSM kernel launch");
     check(hipEventRecord(L.ev[2], g_stream), "hipEventRecord");
     check(hipEventSynchronize(L.ev[2]), "hipEventSynchronize");
-    float train_ms = 0.0f, value_ms = 0.0f;
+    float train_ms = 0.0f, value_ms = 0.0f, path_ms = 0.0f;
     check(hipEventElapsedTime(&train_ms, L.ev[0], L.ev[1]), "hipEventElapsedTime");
+    check(hipEventElapsedTime(&path_ms, L.ev[0], L.ev[3]), "hipEventElapsedTime");
     check(hipEventElapsedTime(&value_ms, L.ev[1], L.ev[2]), "hipEventElapsedTime");
     check(hipMemcpy(L.host_partials.data(), L.partials.ptr, L.n_partials * sizeof(BlockStat), hipMemcpyDeviceToHost),
           "partials copy");
@@ -1090,6 +1091,7 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
         r.in_sample_value = L.host_in_sample[ti];
         r.training_ms = train_ms;
         r.valuation_ms = value_ms;
+        r.path_ms = path_ms;
         r.exercise_dates = t.n_dates;
         for (uint32_t d = 0; d < t.n_dates && d < r.capacity; ++d) {
             if (r.exercise_counts) r.exercise_counts[d] = L.host_counts[t.count_offset + d];

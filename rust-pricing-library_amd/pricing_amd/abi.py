@@ -270,6 +270,7 @@ class pe_lsm_report(C.Structure):
         ("capacity", C.c_uint32),
         ("exercise_counts", C.POINTER(C.c_uint64)),
         ("decisions", C.POINTER(pe_lsm_decision)),
+        ("path_ms", C.c_double),
     ]
 
 

@@ -136,6 +136,7 @@ class LsmReport:
         self.in_sample_value = r.in_sample_value
         self.training_ms = r.training_ms
         self.valuation_ms = r.valuation_ms
+        self.path_ms = r.path_ms
         self.exercise_dates = r.exercise_dates
         self.exercise_counts = list(counts)[:r.exercise_dates]
         self.decisions = list(decisions)[:max(r.exercise_dates - 1, 0)]
