@@ -79,6 +79,8 @@ struct Plan {
     double validation_vol_bump = 0.0;
     double gamma_bump = 0.0;
     bool request_gamma = false;
+    bool request_delta = false;
+    bool request_vega = false;
     bool risk = false;
     Tape tape;
     bool single_terminal = false;
@@ -96,6 +98,7 @@ struct Plan {
     LocalVarianceGrid grid;
     TimeGrid time_grid;
     std::vector<double> node_forwards;
+    std::vector<double> down_forwards, up_forwards;   // runtimes at spot -/+ the validation bump
     std::vector<size_t> obs_nodes;
 
     uint64_t block = 4096;
@@ -280,8 +283,11 @@ static void compile_plan(Plan& p, const pe_request& r, const pe_exec_policy* pol
     if (p.total_variance > 0.0 && independent < 2) throw std::invalid_argument("insufficient sampling units");
 
     p.request_gamma = rk.gamma != 0;
+    p.request_delta = rk.delta != 0;
+    p.request_vega = rk.vega != 0;
     p.risk = rk.delta || rk.vega || p.request_gamma;
-    if (p.local_vol && p.risk) throw std::invalid_argument("local-vol risk is not supported by this oracle path");
+    if (p.local_vol && p.risk && (p.continuous || !p.div.events.empty()))
+        throw std::invalid_argument("local-vol risk with continuous barriers or dividends is not restated");
     const double residual_spot = p.spot - p.div.reserve;
     if (p.request_gamma) {
         p.gamma_bump = rk.gamma_relative ? p.spot * rk.gamma_bump : rk.gamma_bump;
@@ -290,6 +296,20 @@ static void compile_plan(Plan& p, const pe_request& r, const pe_exec_policy* pol
     p.validation_spot_bump = p.request_gamma ? p.gamma_bump
                                              : std::min(p.spot * kValidationRelativeSpotBump, 0.5 * residual_spot);
     p.validation_vol_bump = p.volatility == 0.0 ? 0.0 : std::min(kValidationVolatilityBump, p.volatility * 0.5);
+    if (p.local_vol && (p.request_delta || p.request_gamma)) {
+        // local_vol_bump_runtimes (pathwise_local_vol.rs:24-69): the runtime
+        // recompiled with the market forward at spot -/+ the bump.
+        Forward down = p.market, up = p.market;
+        down.spot = p.spot - p.validation_spot_bump;
+        up.spot = p.spot + p.validation_spot_bump;
+        if (!(down.spot > 0.0)) throw std::invalid_argument("spot bump leaves a non-positive spot");
+        p.down_forwards.clear();
+        p.up_forwards.clear();
+        for (double t : p.time_grid.nodes) {
+            p.down_forwards.push_back(down.forward(t));
+            p.up_forwards.push_back(up.forward(t));
+        }
+    }
     p.single_terminal = p.obs_days.size() == 1 && p.obs_days[0] == pr.expiry_day && !p.obs_has_pre[0];
 }
 
@@ -641,22 +661,45 @@ static void bs_pathwise_values(const Plan& p, const double* z, double* v) {
 }
 
 // ------------------------------------------------------------- LV paths --
-// evolve_path_inner (local_vol.rs:459-559) without dividends; returns states.
-static void lv_evolve(const Plan& p, double initial_f, const double* shocks, std::vector<double>& states) {
+// evolve_path_inner (local_vol.rs:459-559) without dividends; returns states
+// and, when asked, the per-step cache the reverse sweep reads
+// (LocalVolStepCache, local_vol.rs:806-820).
+struct LvStepCache {
+    double local_variance, state_derivative, shock, dt, exponential;
+    Interp interp;
+};
+
+static void lv_evolve(const Plan& p, const std::vector<double>& forwards, double initial_f, const double* shocks,
+                      std::vector<double>& states, std::vector<LvStepCache>* cache = nullptr) {
     const auto& nodes = p.time_grid.nodes;
     const size_t steps = nodes.size() - 1;
     states.resize(nodes.size());
+    if (cache) cache->resize(steps);
     double state = initial_f;
     states[0] = state;
     for (size_t step = 0; step < steps; ++step) {
         const double t = nodes[step];
         const double next_t = nodes[step + 1];
         const double dt = next_t - t;
-        const double x = std::log(state / p.node_forwards[step]);
-        const double variance = p.grid.interpolate(t, x).value;
+        const double x = std::log(state / forwards[step]);
+        const Interp in = p.grid.interpolate(t, x);
+        const double variance = in.value;
         const double lvol = std::sqrt(variance);
-        const double ratio = p.node_forwards[step + 1] / p.node_forwards[step];
+        const double ratio = forwards[step + 1] / forwards[step];
         const double e = ratio * std::exp(-0.5 * variance * dt + lvol * std::sqrt(dt) * shocks[step]);
+        if (cache) {
+            // interpolation_log_moneyness_derivative (local_variance.rs:387-402) / state
+            double slope = 0.0;
+            if (in.boundary == 0) {
+                const size_t nx = p.grid.x_nodes.size();
+                const size_t row = in.t_cell * nx, next = row + nx, left = in.x_cell;
+                const double width = p.grid.x_nodes[left + 1] - p.grid.x_nodes[left];
+                const double lower = (p.grid.values[row + left + 1] - p.grid.values[row + left]) / width;
+                const double upper = (p.grid.values[next + left + 1] - p.grid.values[next + left]) / width;
+                slope = lower * (1.0 - in.t_weight) + upper * in.t_weight;
+            }
+            (*cache)[step] = LvStepCache{variance, slope / state, shocks[step], dt, e, in};
+        }
         state *= e;
         if (!std::isfinite(state) || state <= 0.0) throw std::runtime_error("non-finite local-vol state");
         states[step + 1] = state;
@@ -666,11 +709,83 @@ static void lv_evolve(const Plan& p, double initial_f, const double* shocks, std
 static thread_local std::vector<double> tl_states;
 
 static double lv_price(const Plan& p, const double* shocks) {
-    lv_evolve(p, p.spot, shocks, tl_states);
+    lv_evolve(p, p.node_forwards, p.spot, shocks, tl_states);
     std::vector<double> post(p.obs_nodes.size());
     for (size_t i = 0; i < post.size(); ++i) post[i] = 0.0 * p.spot + 1.0 * tl_states[p.obs_nodes[i]];
     p.tape.forward(post.data(), post.data(), tl_values);
     return p.discount * tl_values[p.tape.output];
+}
+
+// Discounted payoff of a local-vol path (local_vol_discounted_payoff_at_spot_inner,
+// local_vol_valuation.rs:161-206, no dividends: post spot = 0 * S + 1 * f);
+// with `post_adj`, the tape's observation adjoints too.
+static double lv_payoff(const Plan& p, double spot, const std::vector<double>& states, double* post_adj) {
+    const size_t n = p.obs_nodes.size();
+    std::vector<double> post(n), pre_adj(n, 0.0);
+    for (size_t i = 0; i < n; ++i) post[i] = 0.0 * spot + 1.0 * states[p.obs_nodes[i]];
+    p.tape.forward(post.data(), post.data(), tl_values);
+    if (post_adj) p.tape.reverse(tl_values, post_adj, pre_adj.data());
+    return p.discount * tl_values[p.tape.output];
+}
+
+// local_vol_pathwise_values_and_buckets_inner (pathwise_local_vol.rs:127-361)
+// for discrete payoffs: Vega by the reverse sweep over the path
+// (reverse_path_with_variance_seeds, direct_adjoints.rs:97-159, with zero
+// direct variance seeds; reverse_state_adjoints, local_vol.rs:723-803) summed
+// as sum_k adj_k 2 sqrt(v_k) over the grid nodes (Neumaier); Delta and Gamma by
+// re-simulating the same shocks from spot -/+ the validation bump.
+static void lv_pathwise_values(const Plan& p, const double* shocks, double* v) {
+    std::vector<double> states;
+    std::vector<LvStepCache> cache;
+    lv_evolve(p, p.node_forwards, p.spot, shocks, states, &cache);
+    const size_t n = p.obs_nodes.size();
+    std::vector<double> post_adj(n, 0.0);
+    const double price = lv_payoff(p, p.spot, states, post_adj.data());
+    v[PE_C_PRICE] = price;
+    if (p.request_vega) {
+        std::vector<double> adj(states.size(), 0.0);
+        for (size_t i = 0; i < n; ++i) adj[p.obs_nodes[i]] += p.discount * post_adj[i] * 1.0;
+        // zero direct variance seeds: states[i] += 0 * derivative / state
+        const size_t nx = p.grid.x_nodes.size();
+        std::vector<double> grid_adj(p.grid.values.size(), 0.0);
+        for (size_t step = cache.size(); step-- > 0;) {
+            const LvStepCache& c = cache[step];
+            const double next_adj = adj[step + 1];
+            const double log_exponent_adj = next_adj * states[step + 1];
+            const double lvol = std::sqrt(c.local_variance);
+            adj[step] += next_adj * c.exponential;
+            const double variance_adj = log_exponent_adj * (-0.5 * c.dt + c.shock * std::sqrt(c.dt) / (2.0 * lvol));
+            adj[step] += variance_adj * c.state_derivative;
+            // transpose_accumulate (local_variance.rs:110-123)
+            const size_t row = c.interp.t_cell * nx, next = row + nx, left = c.interp.x_cell;
+            const size_t right = std::min(left + 1, nx - 1);
+            const double tl = 1.0 - c.interp.t_weight, tr = c.interp.t_weight;
+            const double xl = 1.0 - c.interp.x_weight, xr = c.interp.x_weight;
+            grid_adj[row + left] += variance_adj * tl * xl;
+            grid_adj[row + right] += variance_adj * tl * xr;
+            grid_adj[next + left] += variance_adj * tr * xl;
+            grid_adj[next + right] += variance_adj * tr * xr;
+        }
+        NeumaierSum vega;
+        for (size_t k = 0; k < grid_adj.size(); ++k) vega.add((0.0 + grid_adj[k]) * 2.0 * std::sqrt(p.grid.values[k]));
+        v[PE_C_VEGA] = vega.total();
+    }
+    if (p.request_delta || p.request_gamma) {
+        const double sb = p.validation_spot_bump;
+        std::vector<double> bumped;
+        lv_evolve(p, p.down_forwards, p.spot - sb, shocks, bumped);
+        const double down = lv_payoff(p, p.spot - sb, bumped, nullptr);
+        lv_evolve(p, p.up_forwards, p.spot + sb, shocks, bumped);
+        const double up = lv_payoff(p, p.spot + sb, bumped, nullptr);
+        const double delta = (up - down) / (2.0 * sb);
+        v[PE_C_DELTA] = delta;
+        v[PE_C_BUMP_DELTA] = delta;
+        if (p.request_gamma) {
+            const double gamma = (up - 2.0 * price + down) / (sb * sb);
+            v[PE_C_GAMMA] = gamma;
+            v[PE_C_BUMP_GAMMA] = gamma;
+        }
+    }
 }
 
 // ------------------------------------------------------------- normals --
@@ -749,7 +864,9 @@ static UnitFn make_unit(const Plan& p, const RqmcPlan* q, const BridgePlan* brid
             pseudo_normals(p, u, z);
         }
         auto eval = [&p](const std::vector<double>& zz, double* out) {
-            if (p.local_vol) {
+            if (p.local_vol && p.risk) {
+                lv_pathwise_values(p, zz.data(), out);
+            } else if (p.local_vol) {
                 out[PE_C_PRICE] = lv_price(p, zz.data());
             } else if (p.risk) {
                 bs_pathwise_values(p, zz.data(), out);
@@ -1345,7 +1462,7 @@ int oracle_evaluate_shared(const pe_request* const* reqs, uint32_t count, const 
                 if (q) lv_rqmc_shocks(*q, bridge.get(), s, u, z); else lv_pseudo_shocks(p0, bridge.get(), u, z);
                 for (int pass = 0; pass < (anti ? 2 : 1); ++pass) {
                     if (pass == 1) for (double& x : z) x = -x;
-                    lv_evolve(p0, p0.spot, z.data(), tl_states);
+                    lv_evolve(p0, p0.node_forwards, p0.spot, z.data(), tl_states);
                     for (uint32_t i = 0; i < count; ++i) {
                         const Plan& p = *plans[i];
                         std::vector<double> post(p.obs_nodes.size());

@@ -8,12 +8,7 @@ import pytest
 from _support import golden, oracle_evaluate
 from pricing_amd import request
 
-UNSUPPORTED = {
-    # local-vol Delta/Gamma/VegaKT risk is not yet in the oracle; the
-    # price-only cases pin the same paths.
-    "pseudo_mc_delta_gamma_vega_vegakt",
-    "rqmc_delta_gamma_vega_vegakt",
-}
+UNSUPPORTED = set()   # local-vol VegaKT buckets are not restated; their Delta/Gamma/Vega are
 
 CASES = [c for c in golden()["replay"]]
 
