@@ -590,6 +590,8 @@ LvLaunch& prepare_lv(std::vector<HostPlan*>& plans, pe_backend backend) {
     if (plans.size() > static_cast<size_t>(kMaxProducts)) throw PlanError(PE_ERR_SIZE_LIMIT, "too many shared plans");
     for (HostPlan* q : plans) {
         if (!q->lv) throw PlanError(PE_ERR_INVALID_ARGUMENT, "shared evaluation needs local-vol plans");
+        if (q->risk && plans.size() > 1)
+            throw PlanError(PE_ERR_INVALID_ARGUMENT, "shared evaluation needs price-only plans");
         if (q->nodes != p0.nodes || q->spot != p0.spot || q->node_forwards != p0.node_forwards ||
             q->lv_v != p0.lv_v || q->lv_x != p0.lv_x || q->lv_t != p0.lv_t ||
             std::memcmp(&q->req.engine, &e, sizeof(e)) != 0)
@@ -756,6 +758,23 @@ LvLaunch& prepare_lv(std::vector<HostPlan*>& plans, pe_backend backend) {
     const size_t bin_bytes = backend == PE_BACKEND_GPU_FP64 ? 0u : ((a.bins * sizeof(uint16_t) + 15u) & ~static_cast<size_t>(15u));
     L.lds = ((head + 15u) & ~static_cast<size_t>(15u)) + bin_bytes +
             static_cast<size_t>(std::max<uint32_t>(a.depth, 1u)) * kBlockThreads * real;
+    if (p0.risk) {
+        // lv_risk_kernel: bumped runtimes and the 2 sqrt(v) node tangents after the price layout
+        L.risk = true;
+        a.risk = (p0.request_vega ? kLvRiskVega : 0u) | (p0.request_gamma ? kLvRiskGamma : 0u) |
+                 (p0.request_delta || p0.request_gamma ? kLvRiskBumps : 0u);
+        a.spot_bump = p0.spot_bump;
+        a.spot_dn = p0.spot - p0.spot_bump;
+        a.spot_up = p0.spot + p0.spot_bump;
+        a.scale_dn = a.spot_dn / p0.spot;
+        a.scale_up = a.spot_up / p0.spot;
+        std::vector<LvBumpStep> bumps = p0.lv_bumps;
+        if (bumps.empty()) bumps.assign(1, LvBumpStep{1.0, 1.0, 1.0, 1.0});
+        L.d_bumps.upload(bumps);
+        a.bump_steps = L.d_bumps.as<LvBumpStep>();
+        a.risk_lds_offset = static_cast<uint32_t>((L.lds + 15u) & ~static_cast<size_t>(15u));
+        L.lds = a.risk_lds_offset + static_cast<size_t>(a.nt) * a.nx * real;
+    }
     if (L.lds > 160u * 1024u) throw PlanError(PE_ERR_SIZE_LIMIT, "local-vol grid and bridge stack exceed LDS");
     p0.lv_launch = std::move(launch);
     return *p0.lv_launch;
@@ -767,9 +786,14 @@ void evaluate_lv(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* o
     *L.host_err = no_error;
     check(hipMemcpyAsync(L.err.ptr, L.host_err, sizeof(int), hipMemcpyHostToDevice, g_stream), "error word init");
     const float ms = timed([&] {
-        check(backend == PE_BACKEND_GPU_FP64 ? launch_lv_f64(L.args, L.grid, L.lds, L.err.as<int>(), g_stream)
-                                             : launch_lv_f32(L.args, L.grid, L.lds, L.err.as<int>(), g_stream),
-              "lv kernel launch");
+        if (L.risk)
+            check(backend == PE_BACKEND_GPU_FP64 ? launch_lv_risk_f64(L.args, L.grid, L.lds, L.err.as<int>(), g_stream)
+                                                 : launch_lv_risk_f32(L.args, L.grid, L.lds, L.err.as<int>(), g_stream),
+                  "lv risk kernel launch");
+        else
+            check(backend == PE_BACKEND_GPU_FP64 ? launch_lv_f64(L.args, L.grid, L.lds, L.err.as<int>(), g_stream)
+                                                 : launch_lv_f32(L.args, L.grid, L.lds, L.err.as<int>(), g_stream),
+                  "lv kernel launch");
     });
     check(hipMemcpyAsync(L.host_partials, L.partials.ptr, L.partial_count * sizeof(BlockStat), hipMemcpyDeviceToHost,
                          g_stream), "partials copy");
@@ -785,7 +809,26 @@ void evaluate_lv(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* o
     const std::vector<BlockStat> partials(host, host + L.partial_count);
     for (size_t k = 0; k < plans.size(); ++k) {
         std::memset(&outs[k], 0, sizeof(pe_result));
-        assemble(*plans[k], partials, L.grid.x, L.grid.y, kMaxProducts, static_cast<uint32_t>(k), 1, &outs[k]);
+        if (L.risk) {
+            // slots: price, Delta, Vega, Gamma; the bump-validation components
+            // repeat Delta and Gamma (pathwise_local_vol.rs:346-355)
+            assemble(*plans[k], partials, L.grid.x, L.grid.y, kMaxProducts, 0, 4, &outs[k]);
+            const HostPlan& q = *plans[k];
+            if (q.request_delta || q.request_gamma) {
+                outs[k].estimates[PE_C_BUMP_DELTA] = outs[k].estimates[PE_C_DELTA];
+                outs[k].component_sum[PE_C_BUMP_DELTA] = outs[k].component_sum[PE_C_DELTA];
+                outs[k].component_mean[PE_C_BUMP_DELTA] = outs[k].component_mean[PE_C_DELTA];
+                outs[k].component_m2[PE_C_BUMP_DELTA] = outs[k].component_m2[PE_C_DELTA];
+            }
+            if (q.request_gamma) {
+                outs[k].estimates[PE_C_BUMP_GAMMA] = outs[k].estimates[PE_C_GAMMA];
+                outs[k].component_sum[PE_C_BUMP_GAMMA] = outs[k].component_sum[PE_C_GAMMA];
+                outs[k].component_mean[PE_C_BUMP_GAMMA] = outs[k].component_mean[PE_C_GAMMA];
+                outs[k].component_m2[PE_C_BUMP_GAMMA] = outs[k].component_m2[PE_C_GAMMA];
+            }
+        } else {
+            assemble(*plans[k], partials, L.grid.x, L.grid.y, kMaxProducts, static_cast<uint32_t>(k), 1, &outs[k]);
+        }
         outs[k].device_ms = ms;
     }
     g_kernel_ms = ms;

@@ -6,6 +6,7 @@
 
 #include "pe_kernels.inc"
 #include "pe_bs.inc"
+#include "pe_lv_risk.inc"
 #include "pe_rough.inc"
 #include "pe_lsm.inc"
 #include "pe_mlp.inc"
@@ -57,6 +58,9 @@ hipError_t launch_lsm_mlp_f32(const LsmArgs& train, uint32_t train_items, const 
 
 hipError_t launch_lv_f32(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s) {
     return launch_lv<float>(a, grid, lds, err, s);
+}
+hipError_t launch_lv_risk_f32(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s) {
+    return launch_lv_risk<float>(a, grid, lds, err, s);
 }
 
 }  // namespace pe

@@ -2,6 +2,7 @@
 // keeps the reference's rounding sequence (no fused multiply-add).
 #include "pe_kernels.inc"
 #include "pe_bs.inc"
+#include "pe_lv_risk.inc"
 #include "pe_rough.inc"
 #include "pe_lsm.inc"
 #include "pe_mlp.inc"
@@ -31,6 +32,9 @@ hipError_t launch_lsm_mlp_f64(const LsmArgs& train, uint32_t train_items, const 
 
 hipError_t launch_lv_f64(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s) {
     return launch_lv<double>(a, grid, lds, err, s);
+}
+hipError_t launch_lv_risk_f64(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s) {
+    return launch_lv_risk<double>(a, grid, lds, err, s);
 }
 
 }  // namespace pe

@@ -106,8 +106,8 @@ double HostCurve::log_discount(double t) const {
 }
 
 // EquityForward::evaluate (crates/pricing/src/market/forward.rs:129-161), no discrete dividends.
-double HostPlan::forward_at(double t) const {
-    const double f = spot * std::exp(dividend_curve.log_discount(t) - discount_curve.log_discount(t));
+double HostPlan::forward_at_spot(double t, double s) const {
+    const double f = s * std::exp(dividend_curve.log_discount(t) - discount_curve.log_discount(t));
     if (!std::isfinite(f) || f <= 0.0) invalid("non-finite forward");
     return f;
 }
@@ -831,8 +831,9 @@ void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* p
         throw PlanError(PE_ERR_INSUFFICIENT_UNITS, "at least two independent sampling units are required");
 
     p.request_gamma = r.risk.gamma != 0;
+    p.request_delta = r.risk.delta != 0;
+    p.request_vega = r.risk.vega != 0;
     p.risk = r.risk.delta || r.risk.vega || p.request_gamma;
-    if (p.lv && p.risk) unsupported("local-vol pathwise risk");
     if (p.risk && !p.lv && pr.kind == PE_PRODUCT_DIGITAL && !d.smoothing)
         invalid("pathwise risk of a discontinuous payoff needs compact-C2 smoothing");
     if (p.risk && !p.lv && pr.kind == PE_PRODUCT_BARRIER && !p.continuous && !d.smoothing)
@@ -843,6 +844,20 @@ void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* p
         invalid("Gamma bump must be positive and below the escrowed spot");
     p.spot_bump = p.request_gamma ? p.gamma_bump : std::min(p.spot * kRelativeSpotBump, 0.5 * residual_spot);
     p.vol_bump = p.vol == 0.0 ? 0.0 : std::min(kVolatilityBump, p.vol * 0.5);
+    if (p.lv && (p.request_delta || p.request_gamma)) {
+        // local_vol_bump_runtimes (crates/pricing/src/engine/risk/pathwise_local_vol.rs:24-69):
+        // the runtime recompiled with the market forward at spot -/+ the bump.
+        const double down = p.spot - p.spot_bump, up = p.spot + p.spot_bump;
+        if (!(down > 0.0)) invalid("the spot bump leaves a non-positive spot");
+        std::vector<double> fdn, fup;
+        for (double t : p.nodes) {
+            fdn.push_back(p.forward_at_spot(t, down));
+            fup.push_back(p.forward_at_spot(t, up));
+        }
+        p.lv_bumps.resize(p.steps.size());
+        for (size_t s = 0; s < p.steps.size(); ++s)
+            p.lv_bumps[s] = LvBumpStep{fdn[s], fdn[s + 1] / fdn[s], fup[s], fup[s + 1] / fup[s]};
+    }
     p.sobol_dims = p.lv ? static_cast<uint32_t>(p.nodes.size() - 1) : static_cast<uint32_t>(p.obs_times.size());
     if (!p.lv) compile_bs_trade(p, monitoring_end);
     if (p.american) compile_lsm_trade(p);

@@ -69,7 +69,8 @@ struct LvLaunch {
     LvArgs args{};
     dim3 grid;
     size_t lds = 0;
-    DeviceBuffer d_steps, d_node_weight, d_ops32, d_ops64, d_bins, partials, err;
+    DeviceBuffer d_steps, d_node_weight, d_ops32, d_ops64, d_bins, d_bumps, partials, err;
+    bool risk = false;               // one plan with Delta/Vega/Gamma: lv_risk_kernel
     void* host_partials = nullptr;   // pinned
     int* host_err = nullptr;         // pinned
     size_t partial_count = 0;
@@ -147,7 +148,7 @@ struct HostPlan {
     double expiry = 0.0, forward = 0.0, discount = 1.0;
     double vol = 0.0, total_variance = 0.0;
     double gamma_bump = 0.0, spot_bump = 0.0, vol_bump = 0.0;
-    bool risk = false, request_gamma = false;
+    bool risk = false, request_gamma = false, request_delta = false, request_vega = false;
     uint64_t block_size = 4096;
 
     // local vol
@@ -158,6 +159,7 @@ struct HostPlan {
     std::vector<double> node_forwards;
     std::vector<uint32_t> obs_nodes;    // node index per observation
     std::vector<LvStep> steps;
+    std::vector<LvBumpStep> lv_bumps;   // risk: runtimes at spot -/+ spot_bump, per step
     std::vector<BridgeOp> program;
     uint32_t bridge_depth = 0;
 
@@ -187,7 +189,8 @@ struct HostPlan {
     // device tables
     DeviceBuffer d_obs, d_sobol, d_steps, d_program, d_grid, d_grid32, d_x, d_x32, d_node_weight;
 
-    double forward_at(double t) const;
+    double forward_at(double t) const { return forward_at_spot(t, spot); }
+    double forward_at_spot(double t, double s) const;
 };
 
 void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* policy);
@@ -211,6 +214,8 @@ hipError_t launch_rough_f64(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 
                             unsigned long long* error, hipStream_t s);
 hipError_t launch_rough_f32(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 grid, uint32_t payoff_items,
                             unsigned long long* error, hipStream_t s);
+hipError_t launch_lv_risk_f64(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s);
+hipError_t launch_lv_risk_f32(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s);
 hipError_t launch_lsm_f64(const LsmArgs& train, uint32_t train_items, const LsmArgs& value, uint32_t value_items,
                           uint32_t basis, hipStream_t s, hipEvent_t paths_done, hipEvent_t between);
 hipError_t launch_lsm_f32(const LsmArgs& train, uint32_t train_items, const LsmArgs& value, uint32_t value_items,

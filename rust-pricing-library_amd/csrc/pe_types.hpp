@@ -342,6 +342,14 @@ struct MlpLsmArgs {
 };
 
 // Kernel arguments of the local-vol Log-Euler kernel.
+// Forward normalizers of the local-vol runtime recompiled at spot -/+ the
+// validation bump, per step (pathwise_local_vol.rs:24-69).
+struct LvBumpStep {
+    double forward_dn, ratio_dn, forward_up, ratio_up;
+};
+
+enum LvRiskFlags : uint32_t { kLvRiskBumps = 1u, kLvRiskVega = 2u, kLvRiskGamma = 4u };
+
 struct LvArgs {
     const LvStep* steps;
     const BridgeOp* program;
@@ -377,6 +385,12 @@ struct LvArgs {
     uint32_t depth;
     uint32_t sobol_dims;
     uint32_t node0_mask;       // products with a fixing at the valuation node
+    // risk kernel only (one plan): bumped runtimes and the node-tangent table
+    const LvBumpStep* bump_steps;
+    double spot_dn, spot_up, spot_bump;
+    double scale_dn, scale_up;     // bumped spot / spot (mixed mode)
+    uint32_t risk;                 // LvRiskFlags
+    uint32_t risk_lds_offset;      // byte offset of the 2 sqrt(v) table in LDS
 };
 
 }  // namespace pe
