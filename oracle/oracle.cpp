@@ -1199,6 +1199,108 @@ static std::vector<double> rqmc_replicates(const std::vector<double>& v, const p
 }
 
 // execute_early_exercise / execute_early_exercise_rqmc (risk/early_exercise.rs:24-300).
+// Fixed-policy Greeks of an American trade (execute_early_exercise,
+// crates/pricing/src/engine/risk/early_exercise.rs:115-170): every valuation
+// path keeps its stopping date; pathwise_aad_for_output
+// (processes/black_scholes.rs:567-644) differentiates the American graph's
+// output at that date (payoff/compile.rs:67-100: max(signed intrinsic, 0) x
+// notional, ties to the intrinsic) and fixed_policy_pathwise_values
+// (:358-413) adds the Gamma and bump-validation evaluations.
+static Aad aad_exercise(const Plan& p, const double* z, double spot, double vol, size_t s, double discount) {
+    bs_observations(p, z, spot, vol, tl_obs);
+    const Obs& o = tl_obs[s];
+    const pe_product& pr = p.req.product;
+    const bool call = pr.side == PE_CALL;
+    const double signed_intrinsic = call ? o.post - pr.strike : pr.strike - o.post;
+    const bool left = signed_intrinsic >= 0.0;
+    const double value = (left ? signed_intrinsic : 0.0) * pr.notional;
+    const double adj = left ? (call ? 1.0 * pr.notional : -(1.0 * pr.notional)) : 0.0;
+    const Coord c = p.obs_coord[s].at_spot(p.spot, spot);
+    const double delta = adj * c.spot_scale * o.f / spot;
+    const double vega = adj * c.b * o.f * (-vol * p.obs_times[s] + o.w);
+    return {discount * value, discount * delta, discount * vega};
+}
+
+static void fixed_policy_values(const Plan& p, const double* z, size_t s, double discount, double* v) {
+    auto eval = [&](double spot, double vol) { return aad_exercise(p, z, spot, vol, s, discount); };
+    const Aad base = eval(p.spot, p.volatility);
+    double gamma = 0.0;
+    if (p.request_gamma) {
+        const double b = p.gamma_bump;
+        gamma = (eval(p.spot + b, p.volatility).delta - eval(p.spot - b, p.volatility).delta) / (2.0 * b);
+    }
+    const double sb = p.validation_spot_bump;
+    const Aad down = eval(p.spot - sb, p.volatility);
+    const Aad up = eval(p.spot + sb, p.volatility);
+    const double bump_delta = (up.price - down.price) / (2.0 * sb);
+    const double bump_gamma = (up.price - 2.0 * base.price + down.price) / (sb * sb);
+    double bump_vega = 0.0;
+    if (p.validation_vol_bump != 0.0) {
+        const double b = p.validation_vol_bump;
+        bump_vega = (eval(p.spot, p.volatility + b).price - eval(p.spot, p.volatility - b).price) / (2.0 * b);
+    }
+    v[PE_C_PRICE] = base.price;
+    v[PE_C_DELTA] = base.delta;
+    v[PE_C_VEGA] = base.vega;
+    v[PE_C_GAMMA] = gamma;
+    v[PE_C_BUMP_DELTA] = bump_delta;
+    v[PE_C_DELTA_DIFFERENCE] = bump_delta - base.delta;
+    v[PE_C_BUMP_VEGA] = bump_vega;
+    v[PE_C_VEGA_DIFFERENCE] = bump_vega - base.vega;
+    v[PE_C_BUMP_GAMMA] = bump_gamma;
+    v[PE_C_GAMMA_DIFFERENCE] = bump_gamma - gamma;
+}
+
+// constant_vol_fixed_policy_{pseudo,rqmc}_statistics (mc/early_exercise.rs:453-599):
+// the risk components of `out` (every one but the price).
+static void american_fixed_policy_risk(const Plan& p, const std::vector<size_t>& stopping,
+                                       const std::vector<double>& discount, pe_result* out) {
+    const pe_engine& e = p.req.engine;
+    const size_t mult = e.antithetic ? 2 : 1;
+    const size_t dates = p.obs_times.size();
+    auto unit = [&](const RqmcPlan* q, uint32_t scramble) {
+        return [&, q, scramble](uint64_t u, double* v) {
+            std::vector<double> z(dates), mate(dates);
+            for (size_t d = 0; d < dates; ++d)
+                z[d] = p.total_variance == 0.0 ? 0.0
+                       : q ? inverse_standard_normal(q->uniform(scramble, u, static_cast<uint32_t>(d)))
+                           : philox_normal(e.seed, u, static_cast<uint32_t>(d), kValuation);
+            const size_t base = ((q ? static_cast<size_t>(scramble) * e.units : 0) + u) * mult;
+            fixed_policy_values(p, z.data(), stopping[base], discount[stopping[base]], v);
+            if (mult == 2) {
+                double m[PE_COMPONENTS] = {0};
+                for (size_t d = 0; d < dates; ++d) mate[d] = -z[d];
+                fixed_policy_values(p, mate.data(), stopping[base + 1], discount[stopping[base + 1]], m);
+                for (int c = 0; c < PE_COMPONENTS; ++c) v[c] = (v[c] + m[c]) * 0.5;
+            }
+        };
+    };
+    std::vector<Statistics> stats;
+    uint64_t independent;
+    if (e.kind == PE_ENGINE_PSEUDO) {
+        stats = run_units(p, e.units, unit(nullptr, 0));
+        independent = e.units;
+    } else {
+        RqmcPlan q;
+        q.compile(g_directions, e.units, e.scrambles, e.seed, static_cast<uint32_t>(dates));
+        std::vector<std::vector<double>> replicate(PE_COMPONENTS);
+        for (uint32_t s = 0; s < e.scrambles; ++s) {
+            const auto within = run_units(p, e.units, unit(&q, s));
+            for (int c = 0; c < PE_COMPONENTS; ++c)
+                replicate[c].push_back(within[c].sum.total() / static_cast<double>(e.units));
+        }
+        stats.resize(PE_COMPONENTS);
+        for (int c = 0; c < PE_COMPONENTS; ++c) stats[c] = Statistics::two_pass(replicate[c]);
+        independent = e.scrambles;
+    }
+    for (int c = PE_C_DELTA; c <= PE_C_GAMMA_DIFFERENCE; ++c) {
+        out->estimates[c] = estimate(stats[c], independent, 1.0);
+        out->component_sum[c] = stats[c].sum.total();
+        out->component_mean[c] = stats[c].moments.mean;
+        out->component_m2[c] = stats[c].moments.m2;
+    }
+}
+
 static void evaluate_american(const Plan& p, pe_result* out, LsmOutcome* keep) {
     const pe_lsm& l = p.req.lsm;
     const pe_engine& ve = p.req.engine;
@@ -1254,6 +1356,10 @@ static void evaluate_american(const Plan& p, pe_result* out, LsmOutcome* keep) {
     out->evaluated_paths = val.paths;
     out->time_steps = static_cast<uint32_t>(dates);
     out->path_steps = val.paths * dates;
+    if (p.risk) {
+        if (p.local_vol) throw std::invalid_argument("local-vol early-exercise Greeks are not restated");
+        american_fixed_policy_risk(p, o.valued.stopping_indices, discount, out);
+    }
     if (keep) *keep = std::move(o);
 }
 

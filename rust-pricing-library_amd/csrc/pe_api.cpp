@@ -849,9 +849,10 @@ struct LsmLaunch {
     std::vector<LsmItem> train_items, value_items;
     std::vector<uint32_t> first_partial, chunks, scrambles;
     DeviceBuffer d_trades, d_dates, d_sobol, d_train_items, d_value_items, store, cash, decisions, counts, in_sample,
-        partials, err;
+        partials, risk_partials, err;
     size_t n_decisions = 0, n_counts = 0, n_partials = 0;
-    std::vector<BlockStat> host_partials;
+    bool risk = false;                       // some plan asks for fixed-policy Greeks
+    std::vector<BlockStat> host_partials, host_risk_partials;
     std::vector<LsmDecision> host_decisions;
     std::vector<unsigned long long> host_counts;
     std::vector<double> host_in_sample;
@@ -1030,6 +1031,11 @@ LsmLaunch& prepare_lsm(std::vector<HostPlan*>& plans, pe_backend backend) {
     L->counts.reserve(L->n_counts * sizeof(unsigned long long));
     L->in_sample.reserve(plans.size() * sizeof(double));
     L->partials.reserve(std::max<size_t>(1, L->n_partials) * sizeof(BlockStat));
+    for (const HostPlan* q : plans) L->risk = L->risk || q->risk;
+    if (L->risk) {
+        L->risk_partials.reserve(std::max<size_t>(1, L->n_partials) * kLsmRiskComponents * sizeof(BlockStat));
+        L->host_risk_partials.resize(L->n_partials * kLsmRiskComponents);
+    }
     L->err.reserve(sizeof(unsigned long long));
     L->host_partials.resize(L->n_partials);
     L->host_decisions.resize(L->n_decisions);
@@ -1056,6 +1062,7 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
     a.counts = L.counts.as<unsigned long long>();
     a.in_sample = L.in_sample.as<double>();
     a.partials = L.partials.ptr;
+    a.risk_partials = L.risk ? L.risk_partials.ptr : nullptr;
     a.error = L.err.as<unsigned long long>();
     a.n_trades = static_cast<uint32_t>(plans.size());
     LsmArgs train = a, value = a;
@@ -1077,15 +1084,15 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
     check(hipEventRecord(L.ev[0], g_stream), "hipEventRecord");
     if (L.mlp)
         check(backend == PE_BACKEND_GPU_FP64
-                  ? launch_lsm_mlp_f64(train, train.n_items, L.mlp_args, L.mlp_wgs, value, value.n_items, g_stream,
+                  ? launch_lsm_mlp_f64(train, train.n_items, L.mlp_args, L.mlp_wgs, value, value.n_items, L.risk, g_stream,
                                        L.ev[3], L.ev[1])
-                  : launch_lsm_mlp_f32(train, train.n_items, L.mlp_args, L.mlp_wgs, value, value.n_items, g_stream,
+                  : launch_lsm_mlp_f32(train, train.n_items, L.mlp_args, L.mlp_wgs, value, value.n_items, L.risk, g_stream,
                                        L.ev[3], L.ev[1]),
               "MLP LSM launch");
     else
         check(backend == PE_BACKEND_GPU_FP64
-                  ? launch_lsm_f64(train, train.n_items, value, value.n_items, L.basis, g_stream, L.ev[3], L.ev[1])
-                  : launch_lsm_f32(train, train.n_items, value, value.n_items, L.basis, g_stream, L.ev[3], L.ev[1]),
+                  ? launch_lsm_f64(train, train.n_items, value, value.n_items, L.basis, L.risk, g_stream, L.ev[3], L.ev[1])
+                  : launch_lsm_f32(train, train.n_items, value, value.n_items, L.basis, L.risk, g_stream, L.ev[3], L.ev[1]),
               "LSM kernel launch");
     check(hipEventRecord(L.ev[2], g_stream), "hipEventRecord");
     check(hipEventSynchronize(L.ev[2]), "hipEventSynchronize");
@@ -1095,6 +1102,10 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
     check(hipEventElapsedTime(&value_ms, L.ev[1], L.ev[2]), "hipEventElapsedTime");
     check(hipMemcpy(L.host_partials.data(), L.partials.ptr, L.n_partials * sizeof(BlockStat), hipMemcpyDeviceToHost),
           "partials copy");
+    if (L.risk)
+        check(hipMemcpy(L.host_risk_partials.data(), L.risk_partials.ptr,
+                        L.host_risk_partials.size() * sizeof(BlockStat), hipMemcpyDeviceToHost),
+              "risk partials copy");
     if (L.n_decisions)
         check(hipMemcpy(L.host_decisions.data(), L.decisions.ptr, L.n_decisions * sizeof(LsmDecision),
                         hipMemcpyDeviceToHost),
@@ -1128,6 +1139,21 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
         pe_result* out = outs + ti;
         std::memset(out, 0, sizeof(*out));
         assemble(p, mine, L.chunks[ti], L.scrambles[ti], 1, 0, 1, out);
+        if (p.risk) {
+            // fixed-policy Greeks: components 1..9 from the valuation kernel's risk partials
+            // (execute_early_exercise, crates/pricing/src/engine/risk/early_exercise.rs:115-170)
+            const size_t first = static_cast<size_t>(L.first_partial[ti]) * kLsmRiskComponents;
+            std::vector<BlockStat> rmine(L.host_risk_partials.begin() + first,
+                                         L.host_risk_partials.begin() + first + static_cast<size_t>(per) * kLsmRiskComponents);
+            pe_result r{};
+            assemble(p, rmine, L.chunks[ti], L.scrambles[ti], kLsmRiskComponents, 0, kLsmRiskComponents, &r);
+            for (int c = 0; c < kLsmRiskComponents; ++c) {
+                out->estimates[1 + c] = r.estimates[c];
+                out->component_sum[1 + c] = r.component_sum[c];
+                out->component_mean[1 + c] = r.component_mean[c];
+                out->component_m2[1 + c] = r.component_m2[c];
+            }
+        }
         out->device_ms = train_ms + value_ms;
         if (!reports) continue;
         pe_lsm_report& r = reports[ti];

@@ -25,8 +25,8 @@ hipError_t launch_rough_f32(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 
 }
 
 hipError_t launch_lsm_f32(const LsmArgs& train, uint32_t train_items, const LsmArgs& value, uint32_t value_items,
-                          uint32_t basis, hipStream_t s, hipEvent_t paths_done, hipEvent_t between) {
-    return launch_lsm<float>(train, train_items, value, value_items, basis, s, paths_done, between);
+                          uint32_t basis, bool risk, hipStream_t s, hipEvent_t paths_done, hipEvent_t between) {
+    return launch_lsm<float>(train, train_items, value, value_items, basis, risk, s, paths_done, between);
 }
 
 hipError_t launch_mlp_gradient(const float* params, const float* x, const float* y, uint32_t rows, uint32_t rows_wg,
@@ -52,8 +52,8 @@ hipError_t launch_mlp_forward(const float* params, const float* x, uint32_t rows
 }
 
 hipError_t launch_lsm_mlp_f32(const LsmArgs& train, uint32_t train_items, const MlpLsmArgs& m, uint32_t wgs,
-                              const LsmArgs& value, uint32_t value_items, hipStream_t s, hipEvent_t paths_done, hipEvent_t between) {
-    return launch_lsm_mlp<float>(train, train_items, m, wgs, value, value_items, s, paths_done, between);
+                              const LsmArgs& value, uint32_t value_items, bool risk, hipStream_t s, hipEvent_t paths_done, hipEvent_t between) {
+    return launch_lsm_mlp<float>(train, train_items, m, wgs, value, value_items, risk, s, paths_done, between);
 }
 
 hipError_t launch_lv_f32(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s) {

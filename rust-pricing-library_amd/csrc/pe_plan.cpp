@@ -526,7 +526,6 @@ void compile_lsm_trade(HostPlan& p) {
     const pe_request& r = p.req;
     const pe_lsm& l = r.lsm;
     if (p.lv) unsupported("early exercise under local volatility on the device");
-    if (p.risk) unsupported("fixed-policy early-exercise Greeks");
     if (l.method != PE_LSM_POLY_CPQR && l.method != PE_LSM_POLY_LAGUERRE && l.method != PE_LSM_MLP)
         invalid("LSM method");
     if (l.max_degree + 1 > static_cast<uint32_t>(kLsmMaxBasis)) unsupported("LSM basis degree above 3 on the device");
@@ -555,6 +554,7 @@ void compile_lsm_trade(HostPlan& p) {
         d.discount = std::exp(p.discount_curve.log_discount(static_cast<double>(p.obs_days[i]) / 365.0));
         d.a = o.a;
         d.b = o.b;
+        d.ss = o.ss;
         d.m_half_tv = -0.5 * (p.vol * p.vol * o.time);
         d.inv_discount = 1.0 / d.discount;
         d.f_sqrt_step = static_cast<float>(d.sqrt_step);
@@ -594,6 +594,11 @@ void compile_lsm_trade(HostPlan& p) {
     t.f_strike = static_cast<float>(t.strike);
     t.f_s0 = static_cast<float>(p.spot);
     t.f_inv_strike = static_cast<float>(t.inv_strike);
+    t.risk = p.risk ? 1u : 0u;
+    t.request_gamma = p.request_gamma ? 1u : 0u;
+    t.gamma_bump = p.gamma_bump;
+    t.spot_bump = p.spot_bump;
+    t.vol_bump = p.vol_bump;
 }
 
 void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* policy) {

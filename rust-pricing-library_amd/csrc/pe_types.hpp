@@ -257,7 +257,7 @@ struct LsmCuts {
 // dividend coordinate at the initial spot, S = a * S0 + b * F.
 struct LsmDate {
     double time, sqrt_step, forward, discount;   // discount = DF(date)
-    double a, b;
+    double a, b, ss;                             // post-dividend coordinate A, B and spot scale
     double m_half_tv;                            // -vol^2 t / 2
     double inv_discount;                         // 1 / DF(date) (mixed targets)
     float f_sqrt_step, f_forward, f_m_half_tv, f_pad;
@@ -284,7 +284,12 @@ struct LsmTrade {
     double strike, notional, s0, vol;
     double abs_tol, rel_tol, itm_tol, inv_strike;
     float f_vol, f_strike, f_s0, f_inv_strike;
+    // fixed-policy Greeks (valuation kernel): Gamma bump, validation spot and vol bumps
+    double gamma_bump, spot_bump, vol_bump;
+    uint32_t risk, request_gamma;
 };
+
+constexpr int kLsmRiskComponents = 9;   // Delta .. Gamma difference (PE component indices 1..9)
 
 // One date's fitted continuation model (ExerciseDecisionModel).
 struct LsmDecision {
@@ -315,6 +320,7 @@ struct LsmArgs {
     unsigned long long* counts;       // valuation exercise counts [trade][date]
     double* in_sample;                // per trade: mean of the training cash flows
     void* partials;                   // BlockStat per valuation item
+    void* risk_partials;              // kLsmRiskComponents BlockStats per valuation item (fixed-policy Greeks)
     unsigned long long* error;
     const LsmCuts* cuts;              // network policy: [trade][non-terminal date]
     uint32_t n_items, n_trades;

@@ -81,6 +81,8 @@ def main():
         "request": c["request"],
         "reduction_block_size": c["plan"]["reduction_block_size"],
         "value": c["result"]["value"],
+        "risks": {k: {"value": v["raw"]["value"], "standard_error": v["raw"]["standard_error"]}
+                  for k, v in (c["result"].get("risks") or {}).items() if isinstance(v, dict) and "raw" in v},
         "early_exercise": {k: v for k, v in c["result"]["monte_carlo"]["early_exercise"].items()
                            if k in ("decision_models", "exercise_counts", "in_sample_value", "stopping_indices")},
     } for c in ee["cases"]]

@@ -79,9 +79,17 @@ def _compare_oracle(req, block=256):
 @pytest.mark.parametrize("case", REPLAY, ids=[c["name"] for c in REPLAY])
 def test_replay_on_device(case):
     req = request.Request.from_v3(case["request"])
-    req.set_risk()   # fixed-policy Greeks are not on the device; the price is the replay's
     dev, rep = _device(req)
     exp = case["value"]
+    # fixed-policy Greeks: pathwise at each path's stopping date; Gamma is a
+    # difference of pathwise Deltas over 2h (rounding scale |Delta| / h)
+    h = req.c.market.spot * req.c.risk.gamma_bump if req.c.risk.gamma else 1.0
+    for key, idx, scale in (("delta", 1, 0.0), ("vega", 2, 0.0), ("gamma", 3, abs(dev.estimates[1].value) / h)):
+        if key not in case["risks"]:
+            continue
+        e = case["risks"][key]
+        assert abs(dev.estimates[idx].value - e["value"]) <= 1e-12 * (abs(e["value"]) + scale), (key, dev.estimates[idx].value, e["value"])
+        assert abs(dev.estimates[idx].standard_error - e["standard_error"]) <= 1e-10 * e["standard_error"], key
     assert abs(dev.estimates[0].value - exp["value"]) <= 1e-12 * abs(exp["value"])
     assert abs(dev.estimates[0].standard_error - exp["standard_error"]) <= 1e-10 * exp["standard_error"]
     ee = case["early_exercise"]
@@ -240,3 +248,69 @@ def test_mlp_policy_captures_the_early_exercise_premium():
     european = 5.573526022256971
     assert v < tree + 3.0 * se
     assert v - european > 0.6 * (tree - european), (v, tree)
+
+
+def _bump_scale(req, ref, c):
+    """Rounding scale of component c: the bump-validation components divide
+    price differences by the spot bump h (h^2) or the vol bump."""
+    price = abs(ref.estimates[0].value) + 1e-300
+    spot = req.c.market.spot
+    h = spot * req.c.risk.gamma_bump if req.c.risk.gamma else spot * 1e-4
+    vb = min(1e-4, 0.5 * req.c.model.volatility)
+    extra = {3: abs(ref.estimates[1].value) / h, 4: price / h, 5: price / h, 6: price / vb, 7: price / vb,
+             8: price / h ** 2, 9: price / h ** 2}
+    return abs(ref.estimates[c].value) + extra.get(c, 0.0)
+
+
+FIXED_POLICY = [
+    ("pseudo-cpqr", dict(method=abi.PE_LSM_POLY_CPQR, units=2048, training_units=2048)),
+    ("rqmc-laguerre", dict(method=abi.PE_LSM_POLY_LAGUERRE, units=512, training_units=512, rqmc=True)),
+    ("dividends-cpqr", dict(method=abi.PE_LSM_POLY_CPQR, units=2048, training_units=2048,
+                            divs=[(0.4, 1.0, 0.01)])),
+]
+
+
+@pytest.mark.parametrize("name,kw", FIXED_POLICY, ids=[f[0] for f in FIXED_POLICY])
+def test_fixed_policy_greeks_fp64_match_oracle(name, kw):
+    req = _put(**kw)
+    req.set_risk(delta=True, vega=True, gamma_bump=0.01)
+    print("case", name, flush=True)
+    dev, _ = _device(req)
+    ref = oracle_lsm(req)[0]
+    for c in range(10):
+        scale = _bump_scale(req, ref, c)
+        assert abs(dev.estimates[c].value - ref.estimates[c].value) <= 1e-12 * scale + 1e-15, (
+            c, dev.estimates[c].value, ref.estimates[c].value)
+        se_ref = ref.estimates[c].standard_error
+        se_scale = se_ref * scale / max(abs(ref.estimates[c].value), 1e-300)
+        assert abs(dev.estimates[c].standard_error - se_ref) <= 1e-10 * max(se_ref, se_scale) + 1e-15, c
+
+
+def test_fixed_policy_greeks_mixed_within_tenth_se():
+    req = _put(abi.PE_LSM_POLY_LAGUERRE, 1 << 14, 1 << 14)
+    req.set_risk(delta=True, vega=True, gamma_bump=0.01)
+    plan = engine.PricingPlan.compile(req)
+    f64 = engine.evaluate_lsm([plan], "fp64")[0][0]
+    mixed = engine.evaluate_lsm([plan], "mixed")[0][0]
+    for c in (0, 1, 2, 3):
+        se = f64.estimates[c].standard_error
+        assert abs(mixed.estimates[c].value - f64.estimates[c].value) <= 0.1 * se, (
+            c, mixed.estimates[c].value, f64.estimates[c].value, se)
+
+
+def test_fixed_policy_delta_matches_the_tree_delta():
+    # The fixed-policy pathwise Delta of a near-optimal policy approximates the
+    # Bermudan Delta (the policy's first-order effect vanishes at the optimum);
+    # CRR tree Delta by central difference of tree prices.
+    from test_oracle_lsm import crr_bermudan_put
+
+    dates = [73, 146, 219, 292, 365]
+    req = _put(abi.PE_LSM_POLY_LAGUERRE, 1 << 16, 1 << 16, vol=0.25, dates=dates)
+    req.set_risk(delta=True)
+    dev, _ = _device(req)
+    times = [d / 365.0 for d in dates]
+    up = crr_bermudan_put(100.5, 100.0, 0.05, 0.25, 1.0, times, steps=2000)
+    dn = crr_bermudan_put(99.5, 100.0, 0.05, 0.25, 1.0, times, steps=2000)
+    tree_delta = (up - dn) / 1.0
+    assert abs(dev.estimates[1].value - tree_delta) < 4.0 * dev.estimates[1].standard_error + 0.005, (
+        dev.estimates[1].value, tree_delta)

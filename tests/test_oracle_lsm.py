@@ -102,6 +102,16 @@ def test_american_replay(case):
             assert abs(row[10 + k] - m["diagonal_abs"][k]) <= 1e-12 * m["diagonal_abs"][k]
         assert abs(row[4] - m["feature_scalings"][0]["mean"]) <= 1e-13 * abs(m["feature_scalings"][0]["mean"])
         assert abs(row[3] - m["residual_sum_squares"]) <= 1e-10 * m["residual_sum_squares"]
+    # fixed-policy Greeks: pathwise at each path's stopping date; Gamma is a
+    # difference of pathwise Deltas over 2h, so its rounding scale is |Delta| / h
+    h = req.c.market.spot * req.c.risk.gamma_bump if req.c.risk.gamma else 1.0
+    delta = res.estimates[1].value
+    for key, idx, scale in (("delta", 1, 0.0), ("vega", 2, 0.0), ("gamma", 3, abs(delta) / h)):
+        if key not in case["risks"]:
+            continue
+        e = case["risks"][key]
+        assert abs(res.estimates[idx].value - e["value"]) <= 1e-12 * (abs(e["value"]) + scale), (key, res.estimates[idx].value, e["value"])
+        assert abs(res.estimates[idx].standard_error - e["standard_error"]) <= 1e-10 * e["standard_error"], key
 
 
 def test_zero_volatility_put_exercises_at_the_first_date():
