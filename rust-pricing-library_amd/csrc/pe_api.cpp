@@ -209,6 +209,16 @@ struct BsLaunch {
     DeviceBuffer d_trades, d_obs, d_sobol, partials, err;
     std::vector<BlockStat> host_partials;
     size_t partial_count = 0;
+    // persistent grid size per group, chosen by the tuner on the first
+    // evaluation; the tuned launches are then captured in one HIP graph
+    uint32_t grid[4] = {0, 0, 0, 0};
+    bool tuned = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    ~BsLaunch() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
 };
 
 void BsLaunchDeleter::operator()(BsLaunch* l) const { delete l; }
@@ -296,30 +306,82 @@ BsLaunch& prepare_bs(std::vector<HostPlan*>& plans, pe_backend backend) {
     return *p0.bs_launch;
 }
 
+// Enqueues the launch of every non-empty group in `mask` with the given
+// persistent grid sizes (0: one workgroup per item).
+uint32_t enqueue_bs_batch(BsLaunch& L, pe_backend backend, const uint32_t (&grid)[4], uint32_t mask = 15u) {
+    uint32_t launches = 0;
+    for (int g = 0; g < 4; ++g) {
+        BsGroup& grp = L.groups[g];
+        if (grp.items.empty() || !((mask >> g) & 1u)) continue;
+        BsBatchArgs a{};
+        a.trades = L.d_trades.as<BsTrade>();
+        a.obs = L.d_obs.as<BsObs>();
+        a.sobol = L.d_sobol.as<uint32_t>();
+        a.items = grp.d_items.as<BsItem>();
+        a.partials = static_cast<BlockStat*>(L.partials.ptr) + grp.partial_base;
+        a.error = static_cast<unsigned long long*>(L.err.ptr);
+        a.n_items = static_cast<uint32_t>(grp.items.size());
+        a.components = grp.components;
+        a.grid = grid[g];
+        check(backend == PE_BACKEND_GPU_FP64 ? launch_bs_batch_f64(a, grp.risk, grp.cont, g_stream)
+                                             : launch_bs_batch_f32(a, grp.risk, grp.cont, g_stream),
+              "bs kernel launch");
+        ++launches;
+    }
+    return launches;
+}
+
+// Launch-shape tuner: the persistent grid size of each group is picked among
+// one workgroup per item and 4, 8 or 16 resident workgroups per CU by timing
+// the group on its own items (results are bit-identical across the
+// candidates).  Batches of fewer than kTuneMinItems items keep one workgroup
+// per item; PE_BATCH_TUNE=0 turns the tuner off.  The tuned launches are then
+// captured into one HIP graph that later evaluations replay.
+constexpr size_t kTuneMinItems = 4096;
+
+void tune_bs_batch(BsLaunch& L, pe_backend backend) {
+    L.tuned = true;
+    const char* env = std::getenv("PE_BATCH_TUNE");
+    if (env && std::strcmp(env, "0") == 0) return;
+    int device = 0, cus = 0;
+    check(hipGetDevice(&device), "hipGetDevice");
+    check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "device attribute");
+    for (int g = 0; g < 4; ++g) {
+        const size_t n = L.groups[g].items.size();
+        if (n < kTuneMinItems) continue;
+        float best = 0.0f;
+        uint32_t best_grid = 0;
+        for (uint32_t per_cu : {0u, 4u, 8u, 16u}) {
+            const uint32_t candidate = per_cu * static_cast<uint32_t>(cus);
+            if (per_cu && candidate >= n) continue;
+            uint32_t grid[4] = {0, 0, 0, 0};
+            grid[g] = candidate;
+            const float ms = timed([&] { enqueue_bs_batch(L, backend, grid, 1u << g); });
+            if (per_cu == 0) best = ms;
+            if (ms < best) {
+                best = ms;
+                best_grid = candidate;
+            }
+        }
+        L.grid[g] = best_grid;
+    }
+}
+
 void evaluate_bs_batch(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* outs) {
     BsLaunch& L = prepare_bs(plans, backend);
     const unsigned long long none = ~0ull;
+    if (!L.tuned) {
+        check(hipMemcpyAsync(L.err.ptr, &none, sizeof(none), hipMemcpyHostToDevice, g_stream), "error word");
+        tune_bs_batch(L, backend);
+        check(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal), "graph capture");
+        enqueue_bs_batch(L, backend, L.grid);
+        check(hipStreamEndCapture(g_stream, &L.graph), "graph capture");
+        check(hipGraphInstantiate(&L.graph_exec, L.graph, nullptr, nullptr, 0), "graph instantiate");
+    }
     check(hipMemcpyAsync(L.err.ptr, &none, sizeof(none), hipMemcpyHostToDevice, g_stream), "error word");
     uint32_t launches = 0;
-    const float ms = timed([&] {
-        for (int g = 0; g < 4; ++g) {
-            BsGroup& grp = L.groups[g];
-            if (grp.items.empty()) continue;
-            BsBatchArgs a{};
-            a.trades = L.d_trades.as<BsTrade>();
-            a.obs = L.d_obs.as<BsObs>();
-            a.sobol = L.d_sobol.as<uint32_t>();
-            a.items = grp.d_items.as<BsItem>();
-            a.partials = static_cast<BlockStat*>(L.partials.ptr) + grp.partial_base;
-            a.error = static_cast<unsigned long long*>(L.err.ptr);
-            a.n_items = static_cast<uint32_t>(grp.items.size());
-            a.components = grp.components;
-            check(backend == PE_BACKEND_GPU_FP64 ? launch_bs_batch_f64(a, grp.risk, grp.cont, g_stream)
-                                                 : launch_bs_batch_f32(a, grp.risk, grp.cont, g_stream),
-                  "bs kernel launch");
-            ++launches;
-        }
-    });
+    for (int g = 0; g < 4; ++g) launches += L.groups[g].items.empty() ? 0u : 1u;
+    const float ms = timed([&] { check(hipGraphLaunch(L.graph_exec, g_stream), "graph launch"); });
     check(hipMemcpy(L.host_partials.data(), L.partials.ptr, L.partial_count * sizeof(BlockStat),
                     hipMemcpyDeviceToHost),
           "partials copy");

@@ -181,6 +181,8 @@ struct BsBatchArgs {
     unsigned long long* error; // smallest (trade << 40 | unit) whose path failed
     uint32_t n_items;
     uint32_t components;       // statistics per item (1, 10 or 18)
+    uint32_t grid;             // persistent workgroups (0: one per item)
+    uint32_t pad;
 };
 
 // ------------------------------------------------------- lifted Heston --
