@@ -1363,6 +1363,153 @@ static void evaluate_american(const Plan& p, pe_result* out, LsmOutcome* keep) {
     if (keep) *keep = std::move(o);
 }
 
+// --------------------------------------- Andersen-Broadie dual bounds --
+// AndersenBroadiePlan (crates/pricing/src/engine/risk/dual.rs:143-455):
+// the LSM policy trained as for pricing drives, from every outer valuation
+// path and exercise date, an inner rollout of N2 (continuation region) or N3
+// (exercise region) independent paths in a private Philox domain; the
+// martingale increments give the dual upper bound of Andersen and Broadie
+// (2004), section 3.
+constexpr uint32_t kDualInnerDomain = 0x4455414cu;   // dual.rs:21
+
+struct DualPath {  // dual.rs:498-532
+    NeumaierSum correction;
+    double gap = -std::numeric_limits<double>::infinity();
+
+    void observe(double payoff, double continuation, bool exercise, bool terminal) {
+        const double value = exercise ? payoff : continuation;
+        const double candidate = (payoff - value) - correction.total();
+        if (!std::isfinite(candidate)) throw std::runtime_error("dual gap candidate");
+        gap = std::max(gap, candidate);
+        if (exercise && !terminal) correction.add(payoff - continuation);
+    }
+};
+
+struct DualConfig {
+    uint32_t continuation_paths, exercise_paths;
+    uint64_t seed;
+    uint64_t stride() const { return std::max(continuation_paths, exercise_paths); }
+};
+
+static LsmSettings lsm_settings(const Plan& p) {
+    const pe_lsm& l = p.req.lsm;
+    LsmSettings s;
+    s.basis.init(1, l.max_degree);
+    s.laguerre = l.method == PE_LSM_POLY_LAGUERRE;
+    s.strike = p.req.product.strike;
+    s.abs_tol = l.abs_rank_tolerance;
+    s.rel_tol = l.rel_rank_tolerance;
+    s.itm_tol = l.itm_abs_tolerance;
+    s.max_elements = l.max_matrix_elements ? l.max_matrix_elements : UINT64_MAX;
+    return s;
+}
+
+static double dual_immediate(const pe_product& pr, double spot) {  // dual.rs:435-445
+    if (!std::isfinite(spot)) throw std::runtime_error("dual inner spot");
+    const double intrinsic = pr.side == PE_CALL ? spot - pr.strike : pr.strike - spot;
+    return pr.notional * std::max(intrinsic, 0.0);
+}
+
+// rollout_mean (dual.rs:371-433)
+static double dual_rollout(const Plan& p, const std::vector<Decision>& decisions, const std::vector<double>& discount,
+                           const DualConfig& cfg, size_t branch, double brownian0, uint64_t trajectory, bool exercise) {
+    const size_t dates = p.obs_times.size();
+    const uint32_t paths = p.volatility == 0.0 ? 1u : (exercise ? cfg.exercise_paths : cfg.continuation_paths);
+    const double vol = p.volatility;
+    const pe_product& pr = p.req.product;
+    NeumaierSum sum;
+    for (uint32_t inner = 0; inner < paths; ++inner) {
+        const uint64_t random_path = trajectory * cfg.stride() + inner;
+        double brownian = brownian0;
+        double previous = p.obs_times[branch];
+        for (size_t d = branch + 1; d < dates; ++d) {
+            const double time = p.obs_times[d];
+            const uint32_t dimension = static_cast<uint32_t>(branch * dates + d);
+            const double normal = vol == 0.0 ? 0.0 : philox_normal(cfg.seed, random_path, dimension, kDualInnerDomain);
+            brownian += std::sqrt(time - previous) * normal;
+            previous = time;
+            const double variance = vol * vol * time;
+            const double canonical = p.obs_forwards[d] * std::exp(-0.5 * variance + vol * brownian);
+            const Coord c = p.obs_coord[d];
+            const double spot = c.a * p.spot + c.b * canonical;
+            const double immediate = dual_immediate(pr, spot);
+            if (d + 1 == dates || decisions[d].should_exercise(immediate, &spot)) {
+                sum.add(immediate * discount[d]);
+                break;
+            }
+        }
+    }
+    return sum.total() / static_cast<double>(paths);
+}
+
+// outer_sample (dual.rs:312-369): [lower, upper, gap] of one outer trajectory.
+static void dual_outer(const Plan& p, const std::vector<Decision>& decisions, const std::vector<double>& discount,
+                       const DualConfig& cfg, const double* z, uint64_t trajectory, double* out) {
+    bs_observations(p, z, p.spot, p.volatility, tl_obs);
+    const std::vector<Obs> obs = tl_obs;
+    const size_t dates = obs.size();
+    const pe_product& pr = p.req.product;
+    bool have_lower = false;
+    double lower = 0.0;
+    DualPath dual;
+    for (size_t d = 0; d < dates; ++d) {
+        const double s = obs[d].post;
+        const double signed_intrinsic = pr.side == PE_CALL ? s - pr.strike : pr.strike - s;
+        const double payoff = (signed_intrinsic >= 0.0 ? signed_intrinsic : 0.0) * pr.notional;
+        const bool terminal = d + 1 == dates;
+        const bool exercise = terminal || decisions[d].should_exercise(payoff, &s);
+        const double discounted = payoff * discount[d];
+        if (exercise && !have_lower) {
+            lower = discounted;
+            have_lower = true;
+        }
+        const double continuation =
+            terminal ? 0.0 : dual_rollout(p, decisions, discount, cfg, d, obs[d].w, trajectory, exercise);
+        dual.observe(discounted, continuation, exercise, terminal);
+    }
+    out[0] = lower;
+    out[1] = lower + dual.gap;
+    out[2] = dual.gap;
+}
+
+// AndersenBroadiePlan::evaluate (dual.rs:219-296): estimates of the lower
+// bound, upper bound and duality gap.
+static void evaluate_dual(const Plan& p, const DualConfig& cfg, pe_estimate* out) {
+    const pe_engine& ve = p.req.engine;
+    const pe_engine& te = p.req.lsm.training_engine;
+    if (!p.american) throw std::invalid_argument("Andersen-Broadie needs an American product");
+    if (p.local_vol) throw std::invalid_argument("Andersen-Broadie supports Black-Scholes/Black-76 only");
+    if (p.risk) throw std::invalid_argument("Andersen-Broadie is price-only");
+    if (ve.kind != PE_ENGINE_PSEUDO || te.kind != PE_ENGINE_PSEUDO)
+        throw std::invalid_argument("Andersen-Broadie needs pseudo-MC training and outer engines");
+    if (cfg.continuation_paths == 0 || cfg.exercise_paths == 0) throw std::invalid_argument("zero inner paths");
+    const LsmSettings s = lsm_settings(p);
+    const size_t dates = p.obs_times.size();
+    std::vector<double> discount;
+    for (uint32_t i = 0; i < p.req.product.date_count; ++i)
+        discount.push_back(p.market.discount_curve.discount(static_cast<double>(p.req.product.dates[i]) / 365.0));
+    TrainedPolicy trained;
+    {
+        const LsmMatrices train = lsm_matrices(p, te, kLsmTrain);
+        trained = train_exercise_policy(s, dates, train.paths, train.immediate, train.features, discount);
+    }
+    const std::vector<Decision>& decisions = trained.decisions;
+    const uint64_t lanes = ve.antithetic ? 2 : 1;
+    const auto stats = map_reduce_statistics_n(ve.units, p.block, p.workers, 3, [&](uint64_t unit, double* v) {
+        std::vector<double> z(dates);
+        for (size_t d = 0; d < dates; ++d)
+            z[d] = p.total_variance == 0.0 ? 0.0 : philox_normal(ve.seed, unit, static_cast<uint32_t>(d), kValuation);
+        dual_outer(p, decisions, discount, cfg, z.data(), unit * lanes, v);
+        if (lanes == 2) {
+            double mate[3];
+            for (double& x : z) x = -x;
+            dual_outer(p, decisions, discount, cfg, z.data(), unit * lanes + 1, mate);
+            for (int i = 0; i < 3; ++i) v[i] = 0.5 * v[i] + 0.5 * mate[i];
+        }
+    });
+    for (int i = 0; i < 3; ++i) out[i] = estimate(stats[i], ve.units, 1.0);
+}
+
 }  // namespace oracle
 
 using namespace oracle;
@@ -1375,6 +1522,20 @@ static int fail(const std::exception& e, int code) {
 extern "C" {
 
 const char* oracle_last_error(void) { return g_error.c_str(); }
+
+// Andersen-Broadie bounds of an American request: out[0] lower bound, out[1]
+// upper bound, out[2] duality gap.
+int oracle_lsm_dual(const pe_request* req, const pe_exec_policy* pol, uint32_t continuation_paths,
+                    uint32_t exercise_paths, uint64_t inner_seed, pe_estimate* out) {
+    try {
+        Plan p;
+        compile_plan(p, *req, pol);
+        evaluate_dual(p, DualConfig{continuation_paths, exercise_paths, inner_seed}, out);
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e, 1);
+    }
+}
 
 // Loads the Joe--Kuo direction asset (JK621201 header, big-endian words).
 int oracle_init(const char* direction_file) {

@@ -69,6 +69,8 @@ def oracle():
     lib.oracle_plan_coordinates.argtypes = [C.POINTER(abi.pe_request), D, C.c_uint32, D]
     lib.oracle_lsm_evaluate.argtypes = [C.POINTER(abi.pe_request), C.POINTER(abi.pe_exec_policy),
                                         C.POINTER(abi.pe_result), D, C.c_uint32, D, C.POINTER(C.c_uint64)]
+    lib.oracle_lsm_dual.argtypes = [C.POINTER(abi.pe_request), C.POINTER(abi.pe_exec_policy), C.c_uint32,
+                                    C.c_uint32, C.c_uint64, C.POINTER(abi.pe_estimate)]
     lib.oracle_cpqr.argtypes = [D, C.c_uint32, C.c_uint32, D, C.c_double, C.c_double, D]
     lib.oracle_basis_exponents.argtypes = [C.c_uint32, C.c_uint32, U, C.c_uint32]
     lib.oracle_feature_scaling.argtypes = [D, C.c_uint32, D]
@@ -139,3 +141,13 @@ def oracle_lsm(req, block=256, workers=4):
     if n < 0:
         raise RuntimeError(oracle().oracle_last_error().decode())
     return res, [list(diag[20 * d:20 * d + 20]) for d in range(n)], ins.value, list(counts)
+
+
+def oracle_lsm_dual(req, continuation_paths, exercise_paths, inner_seed, block=64, workers=4):
+    """The oracle's Andersen-Broadie bounds: (lower, upper, gap) estimates."""
+    from pricing_amd import abi, request
+    out = (abi.pe_estimate * 3)()
+    pol = request.exec_policy(workers, block)
+    if oracle().oracle_lsm_dual(req.pointer(), C.byref(pol), continuation_paths, exercise_paths, inner_seed, out) != 0:
+        raise RuntimeError(oracle().oracle_last_error().decode())
+    return out[0], out[1], out[2]
