@@ -351,6 +351,31 @@ typedef struct pe_lsm_report {
 pe_status pe_lsm_evaluate(const pe_plan* const* plans, uint32_t count, pe_backend backend, pe_result* results,
                           pe_lsm_report* reports);
 
+/* Andersen-Broadie primal-dual bounds of one American plan
+ * (AndersenBroadiePlan::compile / evaluate,
+ * crates/pricing/src/engine/risk/dual.rs:143-296): the plan's LSM policy is
+ * trained as for pricing, the plan's valuation engine supplies the outer
+ * paths, and from every outer path and non-terminal exercise date the policy
+ * drives `continuation_inner_paths` (continuation region) or
+ * `exercise_inner_paths` (exercise region) independent inner rollouts in a
+ * private random domain seeded by `inner_seed`.  Price-only Black-Scholes /
+ * Black-76 plans with pseudo-MC training and valuation engines and a
+ * polynomial policy; FP64 backend. */
+typedef struct pe_dual_result {
+    pe_estimate lower_bound;
+    pe_estimate upper_bound;
+    pe_estimate duality_gap;
+    double price_confidence_interval_95[2];   /* [lower - 1.96 SE, upper + 1.96 SE] */
+    uint64_t outer_trajectories;
+    uint32_t exercise_date_count;
+    uint32_t pad;
+    double device_ms;
+} pe_dual_result;
+
+pe_status pe_lsm_dual_evaluate(const pe_plan* plan, uint32_t continuation_inner_paths,
+                               uint32_t exercise_inner_paths, uint64_t inner_seed, pe_backend backend,
+                               pe_dual_result* out);
+
 /* Continuation network of the MLP policy, exposed for verification (no
  * reference counterpart).  Parameters: 4417 floats, W1[64][2] | b1[64] |
  * W2[64][64] (row = output unit) | b2[64] | w3[64] | b3; inputs x row-major

@@ -1109,6 +1109,87 @@ LsmLaunch& prepare_lsm(std::vector<HostPlan*>& plans, pe_backend backend) {
     return *p0.lsm_launch;
 }
 
+// Andersen-Broadie bounds (AndersenBroadiePlan::evaluate,
+// crates/pricing/src/engine/risk/dual.rs:219-296): the LSM training launch,
+// then the inner-rollout and outer kernels.
+void evaluate_lsm_dual(HostPlan* plan, uint32_t n2, uint32_t n3, uint64_t seed, pe_dual_result* out) {
+    HostPlan& p = *plan;
+    const pe_engine& ve = p.req.engine;
+    const pe_engine& te = p.req.lsm.training_engine;
+    if (!p.american) throw PlanError(PE_ERR_INVALID_ARGUMENT, "Andersen-Broadie needs an American plan");
+    if (p.lv || p.rough) throw PlanError(PE_ERR_UNSUPPORTED, "Andersen-Broadie beyond Black-Scholes/Black-76");
+    if (p.risk) throw PlanError(PE_ERR_UNSUPPORTED, "Greeks in Andersen-Broadie (use a fixed-policy risk request)");
+    if (ve.kind != PE_ENGINE_PSEUDO || te.kind != PE_ENGINE_PSEUDO)
+        throw PlanError(PE_ERR_UNSUPPORTED, "RQMC or mixed training/valuation engines in Andersen-Broadie");
+    if (p.req.lsm.method == PE_LSM_MLP) throw PlanError(PE_ERR_UNSUPPORTED, "the network policy in Andersen-Broadie");
+    if (n2 == 0 || n3 == 0) throw PlanError(PE_ERR_INVALID_ARGUMENT, "inner-path counts must be positive");
+    const uint64_t lanes = ve.antithetic ? 2 : 1;
+    const uint64_t trajectories = ve.units * lanes;
+    const uint64_t stride = std::max(n2, n3);
+    if (trajectories > UINT64_MAX / stride) throw PlanError(PE_ERR_SIZE_LIMIT, "Andersen-Broadie random-coordinate count");
+    const uint32_t n = p.lsm_trade.n_dates;
+    if (static_cast<uint64_t>(n) * n > 0xffffffffull) throw PlanError(PE_ERR_SIZE_LIMIT, "Andersen-Broadie dimensions");
+    const uint64_t pairs = trajectories * (n - 1);
+    if (pairs > 0x7fffffffull) throw PlanError(PE_ERR_SIZE_LIMIT, "too many outer trajectory x date rollouts");
+    std::vector<HostPlan*> plans{plan};
+    LsmLaunch& L = prepare_lsm(plans, PE_BACKEND_GPU_FP64);
+    const unsigned long long none = ~0ull;
+    check(hipMemcpyAsync(L.err.ptr, &none, sizeof(none), hipMemcpyHostToDevice, g_stream), "error word");
+    LsmArgs a{};
+    a.trades = L.d_trades.as<LsmTrade>();
+    a.dates = L.d_dates.as<LsmDate>();
+    a.sobol = L.d_sobol.as<uint32_t>();
+    a.store = L.store.ptr;
+    a.cash = L.cash.ptr;
+    a.decisions = L.decisions.as<LsmDecision>();
+    a.counts = L.counts.as<unsigned long long>();
+    a.in_sample = L.in_sample.as<double>();
+    a.partials = L.partials.ptr;
+    a.error = L.err.as<unsigned long long>();
+    a.n_trades = 1;
+    LsmArgs train = a;
+    train.items = L.d_train_items.as<LsmItem>();
+    train.n_items = static_cast<uint32_t>(L.train_items.size());
+    DeviceBuffer cont, partials;
+    cont.reserve(std::max<uint64_t>(1, pairs) * sizeof(double));
+    const uint32_t blocks = static_cast<uint32_t>((ve.units + kBlockThreads - 1) / kBlockThreads);
+    partials.reserve(static_cast<size_t>(blocks) * 3 * sizeof(BlockStat));
+    LsmDualArgs da{};
+    da.lsm = a;
+    da.cont = cont.as<double>();
+    da.partials = partials.ptr;
+    da.inner_seed = seed;
+    da.stride = stride;
+    da.n2 = n2;
+    da.n3 = n3;
+    da.lanes = static_cast<uint32_t>(lanes);
+    const float ms = timed([&] {
+        check(launch_lsm_f64(train, train.n_items, train, 0, L.basis, false, g_stream, nullptr, nullptr),
+              "LSM training launch");
+        check(launch_lsm_dual_f64(da, pairs, blocks, L.basis, g_stream), "Andersen-Broadie launch");
+    });
+    unsigned long long err = none;
+    check(hipMemcpy(&err, L.err.ptr, sizeof(err), hipMemcpyDeviceToHost), "error copy");
+    if (err != none) throw PlanError(PE_ERR_NONFINITE_STATE, "non-finite early-exercise training path");
+    std::vector<BlockStat> host(static_cast<size_t>(blocks) * 3);
+    check(hipMemcpy(host.data(), partials.ptr, host.size() * sizeof(BlockStat), hipMemcpyDeviceToHost),
+          "partials copy");
+    pe_result r{};
+    assemble(p, host, blocks, 1, 3, 0, 3, &r);
+    for (int c = 0; c < 3; ++c)
+        if (!std::isfinite(r.estimates[c].value)) throw PlanError(PE_ERR_NONFINITE_STATE, "non-finite dual bound");
+    std::memset(out, 0, sizeof(*out));
+    out->lower_bound = r.estimates[0];
+    out->upper_bound = r.estimates[1];
+    out->duality_gap = r.estimates[2];
+    out->price_confidence_interval_95[0] = r.estimates[0].confidence_lower;
+    out->price_confidence_interval_95[1] = r.estimates[1].confidence_upper;
+    out->outer_trajectories = trajectories;
+    out->exercise_date_count = n;
+    out->device_ms = ms;
+    g_kernel_ms = ms;
+}
+
 void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* outs, pe_lsm_report* reports) {
     LsmLaunch& L = prepare_lsm(plans, backend);
     const unsigned long long none = ~0ull;
@@ -1409,6 +1490,18 @@ pe_status pe_lsm_evaluate(const pe_plan* const* plans, uint32_t count, pe_backen
             hp.push_back(&const_cast<pe_plan*>(plans[i])->host);
         }
         evaluate_lsm(hp, backend, results, reports);
+    });
+}
+
+pe_status pe_lsm_dual_evaluate(const pe_plan* plan, uint32_t continuation_inner_paths,
+                               uint32_t exercise_inner_paths, uint64_t inner_seed, pe_backend backend,
+                               pe_dual_result* out) {
+    if (!plan || !out) return fail(PE_ERR_INVALID_ARGUMENT, "null argument");
+    if (backend != PE_BACKEND_GPU_FP64) return fail(PE_ERR_UNSUPPORTED, "Andersen-Broadie runs in the FP64 backend");
+    return guarded([&] {
+        ensure_init();
+        evaluate_lsm_dual(&const_cast<pe_plan*>(plan)->host, continuation_inner_paths, exercise_inner_paths,
+                          inner_seed, out);
     });
 }
 

@@ -5,6 +5,7 @@
 #include "pe_lv_risk.inc"
 #include "pe_rough.inc"
 #include "pe_lsm.inc"
+#include "pe_lsm_dual.inc"
 #include "pe_mlp.inc"
 #include "pe_mlp_lsm.inc"
 #include "pe_plan.hpp"
@@ -32,6 +33,10 @@ hipError_t launch_lsm_mlp_f64(const LsmArgs& train, uint32_t train_items, const 
 
 hipError_t launch_lv_f64(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s) {
     return launch_lv<double>(a, grid, lds, err, s);
+}
+hipError_t launch_lsm_dual_f64(const LsmDualArgs& da, uint64_t pairs, uint32_t outer_blocks, uint32_t basis,
+                               hipStream_t s) {
+    return launch_lsm_dual<double>(da, pairs, outer_blocks, basis, s);
 }
 hipError_t launch_lv_risk_f64(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s) {
     return launch_lv_risk<double>(a, grid, lds, err, s);

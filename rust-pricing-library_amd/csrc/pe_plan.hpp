@@ -214,6 +214,8 @@ hipError_t launch_rough_f64(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 
                             unsigned long long* error, hipStream_t s);
 hipError_t launch_rough_f32(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 grid, uint32_t payoff_items,
                             unsigned long long* error, hipStream_t s);
+hipError_t launch_lsm_dual_f64(const LsmDualArgs& da, uint64_t pairs, uint32_t outer_blocks, uint32_t basis,
+                               hipStream_t s);
 hipError_t launch_lv_risk_f64(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s);
 hipError_t launch_lv_risk_f32(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s);
 hipError_t launch_lsm_f64(const LsmArgs& train, uint32_t train_items, const LsmArgs& value, uint32_t value_items,

@@ -328,6 +328,19 @@ struct LsmArgs {
     uint32_t n_items, n_trades;
 };
 
+// Andersen-Broadie bounds of one trained American trade (the valuation
+// engine supplies the outer paths).
+struct LsmDualArgs {
+    LsmArgs lsm;            // trained trade: trades, dates, decisions (one trade)
+    double* cont;           // [trajectory][non-terminal date] inner rollout means
+    void* partials;         // BlockStat [block][3]: lower bound, upper bound, gap
+    uint64_t inner_seed;
+    uint64_t stride;        // max(N2, N3): inner path index stride per trajectory
+    uint32_t n2, n3;        // continuation / exercise inner paths
+    uint32_t lanes;         // 2 with antithetic outer paths
+    uint32_t pad;
+};
+
 // The learned policy's persistent training launch: one network per exercise
 // date shared by every trade of the launch (inputs moneyness and volatility,
 // output the continuation value per unit strike), trained on the pooled

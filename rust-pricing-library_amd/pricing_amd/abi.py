@@ -261,6 +261,19 @@ class pe_lsm_decision(C.Structure):
     ]
 
 
+class pe_dual_result(C.Structure):
+    _fields_ = [
+        ("lower_bound", pe_estimate),
+        ("upper_bound", pe_estimate),
+        ("duality_gap", pe_estimate),
+        ("price_confidence_interval_95", C.c_double * 2),
+        ("outer_trajectories", C.c_uint64),
+        ("exercise_date_count", C.c_uint32),
+        ("pad", C.c_uint32),
+        ("device_ms", C.c_double),
+    ]
+
+
 class pe_lsm_report(C.Structure):
     _fields_ = [
         ("in_sample_value", C.c_double),
@@ -293,6 +306,8 @@ PE_SYMBOLS = {
     ),
     "pe_lsm_evaluate": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.c_int, C.POINTER(pe_result),
                                   C.POINTER(pe_lsm_report)]),
+    "pe_lsm_dual_evaluate": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint64, C.c_int,
+                                       C.POINTER(pe_dual_result)]),
     "pe_mlp_gradients": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint32,
                                    C.POINTER(C.c_float), _DP]),
     "pe_mlp_forward": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint32, C.POINTER(C.c_float)]),

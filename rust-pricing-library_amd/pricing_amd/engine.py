@@ -161,6 +161,15 @@ def evaluate_lsm(plans, backend="fp64", max_dates=1024):
     return list(results), [LsmReport(r, c, d) for r, (c, d) in zip(reports, keep)]
 
 
+def evaluate_lsm_dual(plan, continuation_inner_paths, exercise_inner_paths, inner_seed, backend="fp64"):
+    """Andersen-Broadie lower/upper bounds of one American plan on the device
+    (AndersenBroadiePlan::evaluate); returns the pe_dual_result."""
+    out = abi.pe_dual_result()
+    _check(load().pe_lsm_dual_evaluate(plan._handle, continuation_inner_paths, exercise_inner_paths, inner_seed,
+                                       BACKENDS[backend], C.byref(out)))
+    return out
+
+
 def local_variance_from_ssvi(theta_times, theta_values, terminal_slope, rho, eta, gamma, time_nodes, x_nodes,
                              floor, cap):
     """Dupire local variance of a StandardSsvi surface, built in FP64 on the GPU."""
