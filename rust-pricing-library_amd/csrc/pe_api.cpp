@@ -729,6 +729,10 @@ LvLaunch& prepare_lv(std::vector<HostPlan*>& plans, pe_backend backend) {
     } else {
         for (size_t k = 0; k < steps.size(); ++k) step_op(kLvDirect, static_cast<int32_t>(k));
     }
+    // the kernels read the pushes before each step up to that step: the
+    // program must end with a step
+    if (ops32.empty() || (ops32.back().code & 3u) == kLvPush)
+        throw PlanError(PE_ERR_INVALID_ARGUMENT, "local-vol program must end with a step");
     for (size_t k = 0; k < plans.size(); ++k) {
         const ProductDesc& d = plans[k]->product;
         ProductHot& h = a.hot[k];
