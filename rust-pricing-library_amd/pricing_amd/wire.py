@@ -1,0 +1,217 @@
+"""Schema-v3 wire: request fingerprints and pricing_result documents.
+
+* ``request_fingerprint(doc)`` -- the reference's ``fingerprint_request``
+  (crates/pricing/src/wire.rs:4144-4212): the request value encoded with
+  type tags and big-endian lengths (objects in key order), prefixed by
+  ``b"pricing/request\\0"`` and the schema version, hashed with BLAKE3-256.
+* ``result_document(request, result)`` -- a ``pricing_result`` document
+  (schemas/v3/pricing_result.schema.json): the value estimate, the requested
+  risks in raw and market-scaled units (build_risk_report,
+  crates/pricing/src/engine/risk/report.rs:255-300, 475-489) and the replay
+  block with the request fingerprint.
+
+BLAKE3 is written here from its published specification (O'Connor, Aumasson,
+Neves, Wilcox-O'Hearn, "BLAKE3", 2020): the ChaCha-derived compression
+function, 1 KiB chunks and the binary chaining-value tree; unkeyed hash,
+32-byte output.
+"""
+import struct
+
+__all__ = ["blake3_256", "encode_value", "request_fingerprint", "estimate_document", "result_document"]
+
+_IV = (0x6A09E667, 0xBB67AE85, 0x3C6EF372, 0xA54FF53A, 0x510E527F, 0x9B05688C, 0x1F83D9AB, 0x5BE0CD19)
+_PERMUTATION = (2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8)
+_CHUNK_START, _CHUNK_END, _PARENT, _ROOT = 1, 2, 4, 8
+_MASK = 0xFFFFFFFF
+_CHUNK = 1024
+_BLOCK = 64
+NORMAL_95 = 1.959963984540054   # crates/pricing/src/engine/plan/simulation.rs:24
+
+
+def _rotr(x, n):
+    return ((x >> n) | (x << (32 - n))) & _MASK
+
+
+def _g(s, a, b, c, d, x, y):
+    s[a] = (s[a] + s[b] + x) & _MASK
+    s[d] = _rotr(s[d] ^ s[a], 16)
+    s[c] = (s[c] + s[d]) & _MASK
+    s[b] = _rotr(s[b] ^ s[c], 12)
+    s[a] = (s[a] + s[b] + y) & _MASK
+    s[d] = _rotr(s[d] ^ s[a], 8)
+    s[c] = (s[c] + s[d]) & _MASK
+    s[b] = _rotr(s[b] ^ s[c], 7)
+
+
+def _compress(cv, m, counter, block_len, flags):
+    s = list(cv) + list(_IV[:4]) + [counter & _MASK, (counter >> 32) & _MASK, block_len, flags]
+    m = list(m)
+    for r in range(7):
+        _g(s, 0, 4, 8, 12, m[0], m[1])
+        _g(s, 1, 5, 9, 13, m[2], m[3])
+        _g(s, 2, 6, 10, 14, m[4], m[5])
+        _g(s, 3, 7, 11, 15, m[6], m[7])
+        _g(s, 0, 5, 10, 15, m[8], m[9])
+        _g(s, 1, 6, 11, 12, m[10], m[11])
+        _g(s, 2, 7, 8, 13, m[12], m[13])
+        _g(s, 3, 4, 9, 14, m[14], m[15])
+        if r < 6:
+            m = [m[i] for i in _PERMUTATION]
+    for i in range(8):
+        s[i] ^= s[i + 8]
+        s[i + 8] ^= cv[i]
+    return s
+
+
+def _words(block):
+    return struct.unpack("<16I", block + b"\0" * (_BLOCK - len(block)))
+
+
+def _chunk_output(data, counter):
+    """Chains a chunk's blocks; returns the compression inputs of its last block."""
+    blocks = [data[i:i + _BLOCK] for i in range(0, len(data), _BLOCK)] or [b""]
+    cv = _IV
+    for i, block in enumerate(blocks):
+        flags = (_CHUNK_START if i == 0 else 0) | (_CHUNK_END if i == len(blocks) - 1 else 0)
+        if i == len(blocks) - 1:
+            return cv, _words(block), counter, len(block), flags
+        cv = _compress(cv, _words(block), counter, _BLOCK, flags)[:8]
+
+
+def _chaining_value(output):
+    return _compress(*output)[:8]
+
+
+def _parent_output(left, right):
+    return _IV, tuple(left) + tuple(right), 0, _BLOCK, _PARENT
+
+
+def blake3_256(data):
+    """BLAKE3 hash of `data` (bytes), 32 bytes as lowercase hex."""
+    data = bytes(data)
+    chunks = [data[i:i + _CHUNK] for i in range(0, len(data), _CHUNK)] or [b""]
+    stack = []
+    for index, chunk in enumerate(chunks[:-1]):
+        cv = _chaining_value(_chunk_output(chunk, index))
+        total = index + 1
+        while total & 1 == 0:      # merge completed subtrees
+            cv = _chaining_value(_parent_output(stack.pop(), cv))
+            total >>= 1
+        stack.append(cv)
+    output = _chunk_output(chunks[-1], len(chunks) - 1)
+    while stack:
+        output = _parent_output(stack.pop(), _chaining_value(output))
+    cv, m, counter, block_len, flags = output
+    return struct.pack("<8I", *_compress(cv, m, counter, block_len, flags | _ROOT)[:8]).hex()
+
+
+def encode_value(value, out=None):
+    """encode_value (wire.rs:4165-4212): tag byte, u64 big-endian payload
+    length, payload; integers as u64/i64, other numbers as f64 bits, objects
+    as (key, value) pairs in key order."""
+    out = bytearray() if out is None else out
+    payload = bytearray()
+    if value is None:
+        tag = 0
+    elif isinstance(value, bool):
+        payload.append(1 if value else 0)
+        tag = 1
+    elif isinstance(value, int):
+        if value >= 0:
+            payload += struct.pack(">Q", value)
+            tag = 2
+        else:
+            payload += struct.pack(">q", value)
+            tag = 3
+    elif isinstance(value, float):
+        if value != value or value in (float("inf"), float("-inf")):
+            raise ValueError("non-finite JSON number")
+        payload += struct.pack(">d", value)
+        tag = 4
+    elif isinstance(value, str):
+        payload += value.encode("utf-8")
+        tag = 5
+    elif isinstance(value, (list, tuple)):
+        payload += struct.pack(">Q", len(value))
+        for item in value:
+            encode_value(item, payload)
+        tag = 6
+    elif isinstance(value, dict):
+        payload += struct.pack(">Q", len(value))
+        for key in sorted(value, key=lambda k: k.encode("utf-8")):
+            encode_value(key, payload)
+            encode_value(value[key], payload)
+        tag = 7
+    else:
+        raise TypeError(f"not a JSON value: {type(value).__name__}")
+    out.append(tag)
+    out += struct.pack(">Q", len(payload))
+    out += payload
+    return out
+
+
+def request_fingerprint(doc):
+    """fingerprint_request for a schema-v3 pricing_request document."""
+    if doc.get("document_kind") != "pricing_request":
+        raise ValueError("not a pricing_request document")
+    data = bytearray(b"pricing/request\0") + struct.pack(">I", int(doc.get("schema_version", 3)))
+    encode_value(doc, data)
+    return "blake3-256:" + blake3_256(bytes(data))
+
+
+def estimate_document(value, standard_error, effective_units, estimator, scale=1.0):
+    """Estimate (estimate_from_statistics, report.rs:454-473) of a component
+    scaled by `scale`: value x scale, SE x |scale|, 95% bracket."""
+    v = value * scale
+    se = standard_error * abs(scale)
+    half = NORMAL_95 * se
+    return {"value": v, "standard_error": se, "confidence_lower": v - half, "confidence_upper": v + half,
+            "estimator": {"type": estimator}, "effective_sampling_units": int(effective_units)}
+
+
+def result_document(doc, result, library_version="0.1.0", platform="mi355x-hip"):
+    """pricing_result document for the request document `doc` priced into the
+    pe_result `result` (price and the Greeks the request asks for)."""
+    engine = doc["engine"]["type"]
+    estimator = "pseudo_monte_carlo" if engine == "pseudo_monte_carlo" else "randomized_quasi_monte_carlo"
+    spot = float(doc["market"]["spot"])
+    e = result.estimates
+
+    def est(c, scale=1.0):
+        # the value block keeps the engine's own 95% bracket
+        if scale == 1.0:
+            x = e[c]
+            return {"value": x.value, "standard_error": x.standard_error, "confidence_lower": x.confidence_lower,
+                    "confidence_upper": x.confidence_upper, "estimator": {"type": estimator},
+                    "effective_sampling_units": int(x.effective_units)}
+        return estimate_document(e[c].value, e[c].standard_error, e[c].effective_units, estimator, scale)
+
+    risk = doc.get("risk", {})
+    risks = {}
+    if risk.get("delta"):
+        risks["delta"] = {"raw": est(1), "raw_unit": {"type": "delta_raw"},
+                          "market_scaled": est(1, spot * 0.01), "market_scaled_unit": {"type": "delta_one_percent_spot"}}
+    if risk.get("gamma"):
+        s = spot * 0.01
+        risks["gamma"] = {"raw": est(3), "raw_unit": {"type": "gamma_raw"},
+                          "market_scaled": est(3, s * s),
+                          "market_scaled_unit": {"type": "gamma_one_percent_spot_squared"}}
+    if risk.get("vega"):
+        risks["vega"] = {"raw": est(2), "raw_unit": {"type": "vega_raw"},
+                         "market_scaled": est(2, 0.01), "market_scaled_unit": {"type": "vega_one_vol_point"}}
+    fingerprint = request_fingerprint(doc)
+    return {
+        "document_kind": "pricing_result",
+        "schema_version": 3,
+        "value": est(0),
+        "risks": risks,
+        "diagnostics": {"warnings": []},
+        "replay": {
+            "schema_version": 3,
+            "request_fingerprint": fingerprint,
+            "library_version": library_version,
+            "platform": platform,
+            "migration": {"original_schema_version": 3, "current_schema_version": 3, "migration_ids": [],
+                          "pre_migration_fingerprint": fingerprint, "post_migration_fingerprint": fingerprint},
+        },
+    }

@@ -47,6 +47,8 @@ def replay(name):
             "evaluated_paths": case["execution"]["evaluated_paths"],
             "risk_validation": case["execution"].get("risk_validation"),
             "barrier_bridge": (case.get("path_diagnostics") or {}).get("barrier_bridge"),
+            "request_fingerprint": result["replay"]["request_fingerprint"],
+            "result_risks": {k: v for k, v in (result.get("risks") or {}).items() if isinstance(v, dict) and "raw" in v},
         })
     return cases
 
@@ -81,6 +83,7 @@ def main():
         "request": c["request"],
         "reduction_block_size": c["plan"]["reduction_block_size"],
         "value": c["result"]["value"],
+        "request_fingerprint": c["result"]["replay"]["request_fingerprint"],
         "risks": {k: {"value": v["raw"]["value"], "standard_error": v["raw"]["standard_error"]}
                   for k, v in (c["result"].get("risks") or {}).items() if isinstance(v, dict) and "raw" in v},
         "early_exercise": {k: v for k, v in c["result"]["monte_carlo"]["early_exercise"].items()
@@ -88,6 +91,12 @@ def main():
     } for c in ee["cases"]]
     with open(os.path.join(ROOT, "fixtures", "early-exercise", "reference-cases-v0.1.json")) as f:
         out["early_exercise_cases"] = json.load(f)
+    # schema-v3 golden documents (request and result wire round trips)
+    out["v3_documents"] = {}
+    for name in ("pricing_request.golden.json", "pricing_request_american.golden.json",
+                 "pricing_result.golden.json", "pricing_result_american.golden.json"):
+        with open(os.path.join(ROOT, "fixtures", "v3", name)) as f:
+            out["v3_documents"][name] = json.load(f)
     with open(os.path.join(HERE, "reference_vectors.json"), "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
         f.write("\n")
