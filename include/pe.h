@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define PE_ABI_VERSION 1
+#define PE_ABI_VERSION 2
 
 /* Error taxonomy mirrors MonteCarloError (crates/pricing/src/api/error.rs:256-300)
  * at the granularity a C caller can act on. */
@@ -110,8 +110,62 @@ typedef enum pe_product_kind {
     PE_PRODUCT_BARRIER = 2,
     PE_PRODUCT_ASIAN = 3,
     PE_PRODUCT_LOOKBACK = 4,
-    PE_PRODUCT_AMERICAN = 5     /* AmericanVanillaSpec: exercise dates in dates[], the last at expiry */
+    PE_PRODUCT_AMERICAN = 5,    /* AmericanVanillaSpec: exercise dates in dates[], the last at expiry */
+    PE_PRODUCT_GRAPH = 6        /* GraphProduct (crates/pricing/src/product/graph_product.rs:40-191):
+                                   SSA payoff graph + cash flows in graph_nodes[] / graph_cashflows[] */
 } pe_product_kind;
+
+/* SourceOpcode (crates/pricing/src/product/graph.rs:89-215); the JSON names of
+ * schemas/product-graph-v2/product_graph.schema.json in comments. */
+typedef enum pe_graph_opcode {
+    PE_GRAPH_LITERAL = 0,            /* literal: value */
+    PE_GRAPH_TERMINAL_SPOT = 1,      /* terminal_spot: underlying_id, observation_day */
+    PE_GRAPH_PRE_DIVIDEND_SPOT = 2,  /* pre_dividend_spot: underlying_id, observation_day */
+    PE_GRAPH_ADD = 3,                /* add: left, right */
+    PE_GRAPH_SUBTRACT = 4,           /* subtract: left, right */
+    PE_GRAPH_MULTIPLY = 5,           /* multiply: left, right */
+    PE_GRAPH_DIVIDE = 6,             /* divide: numerator, denominator */
+    PE_GRAPH_MINIMUM = 7,            /* minimum: left, right */
+    PE_GRAPH_MAXIMUM = 8,            /* maximum: left, right */
+    PE_GRAPH_INDICATOR = 9,          /* indicator: input (1 if input >= 0) */
+    PE_GRAPH_NEGATE = 10,            /* negate: input */
+    PE_GRAPH_BARRIER_HIT_STATE = 11, /* barrier_hit_state: previous, hit_weight */
+    PE_GRAPH_SMOOTH_MINIMUM = 12,    /* smooth_minimum: left, right, value = half width */
+    PE_GRAPH_SMOOTH_MAXIMUM = 13,    /* smooth_maximum: left, right, value = half width */
+    PE_GRAPH_SMOOTH_INDICATOR = 14,  /* smooth_indicator: input, value = half width */
+    PE_GRAPH_LOG = 15,               /* log: input */
+    PE_GRAPH_EXP = 16,               /* exp: input */
+    PE_GRAPH_SELECT = 17,            /* select: condition, then_value, else_value */
+    PE_GRAPH_COMPARE = 18            /* compare: left, right, comparison */
+} pe_graph_opcode;
+
+/* GraphComparison (graph.rs:68-87). */
+typedef enum pe_graph_comparison {
+    PE_GRAPH_LESS = 0,
+    PE_GRAPH_LESS_EQUAL = 1,
+    PE_GRAPH_EQUAL = 2,
+    PE_GRAPH_NOT_EQUAL = 3,
+    PE_GRAPH_GREATER_EQUAL = 4,
+    PE_GRAPH_GREATER = 5
+} pe_graph_comparison;
+
+/* One source node (SourceNode, graph.rs:217-236).  operands[] hold node ids
+ * in the order of the opcode's comment above; unused entries are ignored. */
+typedef struct pe_graph_node {
+    uint32_t id;
+    pe_graph_opcode opcode;
+    uint32_t operands[3];
+    pe_graph_comparison comparison;
+    uint32_t underlying_id;
+    int32_t observation_day;         /* days after valuation */
+    double value;                    /* literal value or smoothing half width */
+} pe_graph_node;
+
+/* Cash flow of a graph product: output node and its payment day. */
+typedef struct pe_graph_cashflow {
+    uint32_t output_node;
+    int32_t payment_day;
+} pe_graph_cashflow;
 
 typedef enum pe_side { PE_CALL = 0, PE_PUT = 1 } pe_side;
 
@@ -145,6 +199,14 @@ typedef struct pe_product {
     const int32_t* dates;
     const double* weights;
     const double* fixings;
+    /* Graph (PE_PRODUCT_GRAPH): nodes, cash flows and the declared
+     * underlyings; observations must be on the first (the market's). */
+    uint32_t graph_node_count;
+    const pe_graph_node* graph_nodes;
+    uint32_t graph_cashflow_count;
+    const pe_graph_cashflow* graph_cashflows;
+    uint32_t graph_underlying_count;
+    const uint32_t* graph_underlyings;
 } pe_product;
 
 /* ---------------------------------------------------------------- engine -- */

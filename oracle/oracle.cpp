@@ -41,6 +41,7 @@
 #include "oracle_market.hpp"
 #include "oracle_numerics.hpp"
 #include "oracle_payoff.hpp"
+#include "oracle_graph.hpp"
 #include "oracle_rough.hpp"
 #include "oracle_sampling.hpp"
 #include "pe.h"
@@ -83,6 +84,8 @@ struct Plan {
     bool request_vega = false;
     bool risk = false;
     Tape tape;
+    bool graph = false;                // graph product: `graph_payoff` is the payoff
+    GraphPayoff graph_payoff;
     bool single_terminal = false;
     std::vector<Coord> obs_coord;      // post-dividend affine coordinate per observation
     std::vector<Coord> obs_pre;        // pre-dividend coordinate (valid where obs_has_pre)
@@ -150,10 +153,20 @@ static void compile_plan(Plan& p, const pe_request& r, const pe_exec_policy* pol
     const std::vector<Dividends::Entry> timeline = p.div.timeline();
 
     const pe_product& pr = p.req.product;
+    p.graph = pr.kind == PE_PRODUCT_GRAPH;
+    if (p.graph) {
+        // the graph fixes the expiry (its last observation) and the
+        // discounting date (its last payment)
+        if (r.model.kind == PE_MODEL_LOCAL_VOL) throw std::invalid_argument("graph products under local volatility are not restated");
+        if (r.risk.delta || r.risk.vega || r.risk.gamma) throw std::invalid_argument("graph product Greeks are not restated");
+        p.graph_payoff.compile(pr, [&](int32_t day) { return p.market.discount_curve.discount(static_cast<double>(day) / 365.0); });
+        p.req.product.expiry_day = p.graph_payoff.expiry_day;
+        p.req.product.payment_day = p.graph_payoff.last_payment;
+    }
     p.continuous = pr.kind == PE_PRODUCT_BARRIER && pr.continuous;
     if (p.continuous && r.model.kind == PE_MODEL_LOCAL_VOL)
         throw std::invalid_argument("continuous barriers under local volatility are not supported by this oracle path");
-    const std::vector<int32_t> contractual = observation_days(pr);
+    const std::vector<int32_t> contractual = p.graph ? p.graph_payoff.days : observation_days(pr);
     // Observation points (compile/simulation.rs:210-265): contractual dates,
     // and for continuous barriers the dividend ex-times up to the monitoring end.
     std::vector<std::pair<double, int32_t>> points;
@@ -204,7 +217,8 @@ static void compile_plan(Plan& p, const pe_request& r, const pe_exec_policy* pol
     // A continuous barrier is priced by its bridge runtime and an American by
     // its exercise policy, not by the single-output graph.
     p.american = pr.kind == PE_PRODUCT_AMERICAN;
-    if (!p.continuous && !p.american) p.tape = build_tape(pr, idx, p.smoothing, rk.smoothing_half_width, jump_obs);
+    if (!p.continuous && !p.american && !p.graph)
+        p.tape = build_tape(pr, idx, p.smoothing, rk.smoothing_half_width, jump_obs);
 
     // Affine coordinates per observation (compile/simulation.rs:308-338).
     p.obs_coord.clear();
@@ -216,7 +230,9 @@ static void compile_plan(Plan& p, const pe_request& r, const pe_exec_policy* pol
         const Dividends::Entry* entry = nullptr;
         for (const auto& e : timeline)
             if (e.ex_time == p.obs_times[i]) entry = &e;
-        const bool has_pre = entry && (p.continuous || jump_obs[i]);
+        const bool graph_pre = p.graph && std::binary_search(p.graph_payoff.pre_days.begin(),
+                                                             p.graph_payoff.pre_days.end(), p.obs_days[i]);
+        const bool has_pre = entry && (p.continuous || jump_obs[i] || graph_pre);
         p.obs_has_pre.push_back(has_pre ? 1 : 0);
         p.obs_pre.push_back(has_pre ? entry->before : Coord{});
     }
@@ -620,6 +636,7 @@ static double bs_price(const Plan& p, const double* z) {
         post[i] = tl_obs[i].post;
         pre[i] = tl_obs[i].pre;
     }
+    if (p.graph) return p.discount * p.graph_payoff.evaluate(post.data(), pre.data(), p.obs_has_pre.data());
     p.tape.forward(post.data(), pre.data(), tl_values);
     return p.discount * tl_values[p.tape.output];
 }
@@ -1538,6 +1555,34 @@ int oracle_lsm_dual(const pe_request* req, const pe_exec_policy* pol, uint32_t c
 }
 
 // Loads the Joe--Kuo direction asset (JK621201 header, big-endian words).
+// One lazy evaluation of a graph product's output on given observation
+// values at `days` (flat discounting).  Returns 0, 2 for a compile error or
+// an observation day without a value, 3 for an evaluation error.
+int oracle_graph_evaluate(const pe_product* product, uint32_t n, const int32_t* days, const double* post,
+                          const double* pre, const char* has_pre, double* out) {
+    GraphPayoff g;
+    try {
+        g.compile(*product, [](int32_t) { return 1.0; });
+    } catch (const std::exception& e) {
+        return fail(e, 2);
+    }
+    std::vector<double> gp, gq;
+    std::vector<char> gh;
+    for (int32_t d : g.days) {
+        const int32_t* it = std::find(days, days + n, d);
+        if (it == days + n) return fail(std::invalid_argument("observation day without a value"), 2);
+        gp.push_back(post[it - days]);
+        gq.push_back(pre[it - days]);
+        gh.push_back(has_pre[it - days]);
+    }
+    try {
+        *out = g.evaluate(gp.data(), gq.data(), gh.data());
+    } catch (const std::exception& e) {
+        return fail(e, 3);
+    }
+    return 0;
+}
+
 int oracle_init(const char* direction_file) {
     std::ifstream f(direction_file, std::ios::binary);
     if (!f) { g_error = "cannot open direction file"; return 1; }

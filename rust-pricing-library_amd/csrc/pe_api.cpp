@@ -183,11 +183,14 @@ float timed(F&& launch) {
 // longest-first; each has a fixed partial slot so the per-trade merge below
 // is in unit order whatever the schedule.
 constexpr uint64_t kItemUnits = 4096;
+// groups: bit 0 risk, bit 1 continuous barrier; group 4: graph products
+constexpr int kBsGroups = 5;
 
 }  // namespace
 
 struct BsGroup {
     bool risk = false, cont = false;
+    uint32_t graph_regs = 0;        // graph group: registers of its largest tape
     uint32_t components = 1;
     std::vector<BsItem> items;
     size_t partial_base = 0;        // BlockStats before this group's slots
@@ -205,13 +208,13 @@ struct BsLaunch {
     int backend = 0;
     std::vector<BsTrade> trades;
     std::vector<BsPlanSlot> slots;
-    BsGroup groups[4];
-    DeviceBuffer d_trades, d_obs, d_sobol, partials, err;
+    BsGroup groups[kBsGroups];
+    DeviceBuffer d_trades, d_obs, d_sobol, partials, err, d_graph_ops, d_graph_segs, d_graph_consts;
     std::vector<BlockStat> host_partials;
     size_t partial_count = 0;
     // persistent grid size per group, chosen by the tuner on the first
     // evaluation; the tuned launches are then captured in one HIP graph
-    uint32_t grid[4] = {0, 0, 0, 0};
+    uint32_t grid[kBsGroups] = {0, 0, 0, 0, 0};
     bool tuned = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -239,12 +242,15 @@ BsLaunch& prepare_bs(std::vector<HostPlan*>& plans, pe_backend backend) {
     L->backend = static_cast<int>(backend);
     std::vector<BsObs> obs;
     std::vector<uint32_t> sobol;
-    for (int g = 0; g < 4; ++g) {
-        L->groups[g].risk = (g & 1) != 0;
-        L->groups[g].cont = (g & 2) != 0;
+    std::vector<GraphOp> graph_ops;
+    std::vector<uint32_t> graph_segs;
+    std::vector<double> graph_consts;
+    for (int g = 0; g < kBsGroups; ++g) {
+        L->groups[g].risk = g < 4 && (g & 1) != 0;
+        L->groups[g].cont = g < 4 && (g & 2) != 0;
         L->groups[g].components = L->groups[g].cont ? PE_COMPONENTS : (L->groups[g].risk ? 10u : 1u);
     }
-    uint32_t group_partials[4] = {0, 0, 0, 0};
+    uint32_t group_partials[kBsGroups] = {0, 0, 0, 0, 0};
     for (size_t ti = 0; ti < plans.size(); ++ti) {
         HostPlan& p = *plans[ti];
         if (p.lv) throw PlanError(PE_ERR_INVALID_ARGUMENT, "local-vol plans are evaluated by pe_plans_evaluate_shared");
@@ -259,8 +265,19 @@ BsLaunch& prepare_bs(std::vector<HostPlan*>& plans, pe_backend backend) {
             if (sobol.size() + tab.size() > 0xffffffffull) throw PlanError(PE_ERR_SIZE_LIMIT, "Sobol tables");
             sobol.insert(sobol.end(), tab.begin(), tab.end());
         }
+        if (p.graph) {
+            const GraphTape& gt = p.graph_tape;
+            t.graph_ops = static_cast<uint32_t>(graph_ops.size());
+            t.graph_segs = static_cast<uint32_t>(graph_segs.size());
+            t.graph_consts = static_cast<uint32_t>(graph_consts.size());
+            graph_ops.insert(graph_ops.end(), gt.ops.begin(), gt.ops.end());
+            graph_segs.insert(graph_segs.end(), gt.segs.begin(), gt.segs.end());
+            graph_consts.insert(graph_consts.end(), gt.consts.begin(), gt.consts.end());
+            if (gt.segs.size() != p.bs_obs.size() + 2) throw PlanError(PE_ERR_INVALID_ARGUMENT, "graph schedule");
+            L->groups[4].graph_regs = std::max<uint32_t>(L->groups[4].graph_regs, gt.regs);
+        }
         L->trades.push_back(t);
-        const int g = (p.risk ? 1 : 0) | (p.continuous ? 2 : 0);
+        const int g = p.graph ? 4 : (p.risk ? 1 : 0) | (p.continuous ? 2 : 0);
         BsGroup& grp = L->groups[g];
         BsPlanSlot slot;
         slot.group = g;
@@ -280,7 +297,7 @@ BsLaunch& prepare_bs(std::vector<HostPlan*>& plans, pe_backend backend) {
         L->slots.push_back(slot);
     }
     size_t base = 0;
-    for (int g = 0; g < 4; ++g) {
+    for (int g = 0; g < kBsGroups; ++g) {
         BsGroup& grp = L->groups[g];
         grp.partial_base = base;
         base += static_cast<size_t>(group_partials[g]) * grp.components;
@@ -299,6 +316,11 @@ BsLaunch& prepare_bs(std::vector<HostPlan*>& plans, pe_backend backend) {
     L->d_trades.upload(L->trades);
     L->d_obs.upload(obs);
     if (!sobol.empty()) L->d_sobol.upload(sobol);
+    if (!graph_ops.empty()) {
+        L->d_graph_ops.upload(graph_ops);
+        L->d_graph_segs.upload(graph_segs);
+        L->d_graph_consts.upload(graph_consts);
+    }
     L->partials.reserve(std::max<size_t>(1, base) * sizeof(BlockStat));
     L->err.reserve(sizeof(unsigned long long));
     L->host_partials.resize(base);
@@ -308,9 +330,9 @@ BsLaunch& prepare_bs(std::vector<HostPlan*>& plans, pe_backend backend) {
 
 // Enqueues the launch of every non-empty group in `mask` with the given
 // persistent grid sizes (0: one workgroup per item).
-uint32_t enqueue_bs_batch(BsLaunch& L, pe_backend backend, const uint32_t (&grid)[4], uint32_t mask = 15u) {
+uint32_t enqueue_bs_batch(BsLaunch& L, pe_backend backend, const uint32_t (&grid)[kBsGroups], uint32_t mask = 31u) {
     uint32_t launches = 0;
-    for (int g = 0; g < 4; ++g) {
+    for (int g = 0; g < kBsGroups; ++g) {
         BsGroup& grp = L.groups[g];
         if (grp.items.empty() || !((mask >> g) & 1u)) continue;
         BsBatchArgs a{};
@@ -320,11 +342,14 @@ uint32_t enqueue_bs_batch(BsLaunch& L, pe_backend backend, const uint32_t (&grid
         a.items = grp.d_items.as<BsItem>();
         a.partials = static_cast<BlockStat*>(L.partials.ptr) + grp.partial_base;
         a.error = static_cast<unsigned long long*>(L.err.ptr);
+        a.graph_ops = L.d_graph_ops.as<GraphOp>();
+        a.graph_segs = L.d_graph_segs.as<uint32_t>();
+        a.graph_consts = L.d_graph_consts.as<double>();
         a.n_items = static_cast<uint32_t>(grp.items.size());
         a.components = grp.components;
         a.grid = grid[g];
-        check(backend == PE_BACKEND_GPU_FP64 ? launch_bs_batch_f64(a, grp.risk, grp.cont, g_stream)
-                                             : launch_bs_batch_f32(a, grp.risk, grp.cont, g_stream),
+        check(backend == PE_BACKEND_GPU_FP64 ? launch_bs_batch_f64(a, grp.risk, grp.cont, grp.graph_regs, g_stream)
+                                             : launch_bs_batch_f32(a, grp.risk, grp.cont, grp.graph_regs, g_stream),
               "bs kernel launch");
         ++launches;
     }
@@ -346,7 +371,7 @@ void tune_bs_batch(BsLaunch& L, pe_backend backend) {
     int device = 0, cus = 0;
     check(hipGetDevice(&device), "hipGetDevice");
     check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "device attribute");
-    for (int g = 0; g < 4; ++g) {
+    for (int g = 0; g < kBsGroups; ++g) {
         const size_t n = L.groups[g].items.size();
         if (n < kTuneMinItems) continue;
         float best = 0.0f;
@@ -354,7 +379,7 @@ void tune_bs_batch(BsLaunch& L, pe_backend backend) {
         for (uint32_t per_cu : {0u, 4u, 8u, 16u}) {
             const uint32_t candidate = per_cu * static_cast<uint32_t>(cus);
             if (per_cu && candidate >= n) continue;
-            uint32_t grid[4] = {0, 0, 0, 0};
+            uint32_t grid[kBsGroups] = {0, 0, 0, 0, 0};
             grid[g] = candidate;
             const float ms = timed([&] { enqueue_bs_batch(L, backend, grid, 1u << g); });
             if (per_cu == 0) best = ms;
@@ -380,13 +405,18 @@ void evaluate_bs_batch(std::vector<HostPlan*>& plans, pe_backend backend, pe_res
     }
     check(hipMemcpyAsync(L.err.ptr, &none, sizeof(none), hipMemcpyHostToDevice, g_stream), "error word");
     uint32_t launches = 0;
-    for (int g = 0; g < 4; ++g) launches += L.groups[g].items.empty() ? 0u : 1u;
+    for (int g = 0; g < kBsGroups; ++g) launches += L.groups[g].items.empty() ? 0u : 1u;
     const float ms = timed([&] { check(hipGraphLaunch(L.graph_exec, g_stream), "graph launch"); });
     check(hipMemcpy(L.host_partials.data(), L.partials.ptr, L.partial_count * sizeof(BlockStat),
                     hipMemcpyDeviceToHost),
           "partials copy");
     unsigned long long err = none;
     check(hipMemcpy(&err, L.err.ptr, sizeof(err), hipMemcpyDeviceToHost), "error copy");
+    if (err != none && (err >> 63)) {
+        const uint64_t trade = (err >> 40) & ((1ull << 23) - 1), unit = err & ((1ull << 40) - 1);
+        throw PlanError(PE_ERR_NONFINITE_STATE, "trade " + std::to_string(trade) + ", sampling unit " +
+                                                    std::to_string(unit) + ": non-finite graph payoff value");
+    }
     if (err != none) {
         const uint64_t trade = err >> 40, unit = err & ((1ull << 40) - 1);
         throw PlanError(PE_ERR_NONFINITE_STATE, "trade " + std::to_string(trade) + ", sampling unit " +

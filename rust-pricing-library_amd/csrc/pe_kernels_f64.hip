@@ -12,8 +12,8 @@
 
 namespace pe {
 
-hipError_t launch_bs_batch_f64(const BsBatchArgs& a, bool risk, bool cont, hipStream_t s) {
-    return launch_bs_batch<double>(a, risk, cont, s);
+hipError_t launch_bs_batch_f64(const BsBatchArgs& a, bool risk, bool cont, uint32_t graph_regs, hipStream_t s) {
+    return launch_bs_batch<double>(a, risk, cont, graph_regs, s);
 }
 
 hipError_t launch_rough_f64(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 grid, uint32_t payoff_items,

@@ -332,7 +332,11 @@ void compile_bs_trade(HostPlan& p, double monitoring_end) {
         if (entry && pr.kind == PE_PRODUCT_BARRIER && !p.continuous && p.obs_days[i] != kNoDay)
             for (uint32_t k = 0; k < pr.date_count; ++k)
                 if (pr.dates[k] == p.obs_days[i]) jump = true;
-        if (entry && (p.continuous || jump)) {
+        const bool graph_pre = p.graph && p.obs_days[i] != kNoDay &&
+                               std::binary_search(p.graph_tape.pre_days.begin(), p.graph_tape.pre_days.end(),
+                                                  p.obs_days[i]);
+        if (graph_pre && !entry) unsupported("pre-dividend graph observation on a date without a dividend");
+        if (entry && (p.continuous || jump || graph_pre)) {
             p.obs_flags[i] |= kObsPre;
             p.obs_pre[i] = entry->before;
             has_div = true;
@@ -397,6 +401,8 @@ void compile_bs_trade(HostPlan& p, double monitoring_end) {
     t.resolved = p.resolved ? 1u : 0u;
     t.expiry_obs = static_cast<uint32_t>(p.expiry_obs);
     t.risk = p.risk ? 1u : 0u;
+    t.graph_output = p.graph_tape.output;
+    t.graph_regs = p.graph_tape.regs;
     auto variant = [&](int k, double spot, double vol) { t.variants[k] = Variant{spot, vol, spot / p.spot}; };
     variant(0, p.spot, p.vol);
     uint32_t nv = 1;
@@ -615,6 +621,18 @@ void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* p
     p.req.product.dates = p.dates.empty() ? nullptr : p.dates.data();
     p.req.product.weights = p.weights.empty() ? nullptr : p.weights.data();
     p.req.product.fixings = p.fixings.empty() ? nullptr : p.fixings.data();
+    p.graph = src.kind == PE_PRODUCT_GRAPH;
+    if (p.graph) {
+        if (src.graph_node_count && src.graph_nodes)
+            p.graph_nodes.assign(src.graph_nodes, src.graph_nodes + src.graph_node_count);
+        if (src.graph_cashflow_count && src.graph_cashflows)
+            p.graph_cashflows.assign(src.graph_cashflows, src.graph_cashflows + src.graph_cashflow_count);
+        if (src.graph_underlying_count && src.graph_underlyings)
+            p.graph_underlyings.assign(src.graph_underlyings, src.graph_underlyings + src.graph_underlying_count);
+    }
+    p.req.product.graph_nodes = p.graph_nodes.empty() ? nullptr : p.graph_nodes.data();
+    p.req.product.graph_cashflows = p.graph_cashflows.empty() ? nullptr : p.graph_cashflows.data();
+    p.req.product.graph_underlyings = p.graph_underlyings.empty() ? nullptr : p.graph_underlyings.data();
     const pe_product& pr = p.req.product;
     if (policy && policy->reduction_block_size) p.block_size = policy->reduction_block_size;
 
@@ -631,10 +649,21 @@ void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* p
 
     p.continuous = pr.kind == PE_PRODUCT_BARRIER && pr.continuous;
     if (p.continuous && r.model.kind == PE_MODEL_LOCAL_VOL) unsupported("continuous barriers under local volatility");
-    if (!(pr.strike > 0.0) || !(pr.notional > 0.0)) invalid("strike and notional must be positive");
+    if (p.graph) {
+        // the tape fixes the observation dates, the expiry (last observation)
+        // and the discounting date (last payment)
+        if (r.model.kind == PE_MODEL_LOCAL_VOL) unsupported("graph products under local volatility");
+        if (r.risk.delta || r.risk.vega || r.risk.gamma) unsupported("pathwise Greeks of graph products");
+        compile_graph(pr, [&](int32_t day) { return std::exp(p.discount_curve.log_discount(static_cast<double>(day) / 365.0)); },
+                      p.graph_tape);
+        p.req.product.expiry_day = p.graph_tape.expiry_day;
+        p.req.product.payment_day = p.graph_tape.last_payment;
+    } else if (!(pr.strike > 0.0) || !(pr.notional > 0.0)) {
+        invalid("strike and notional must be positive");
+    }
     for (size_t i = 1; i < p.dates.size(); ++i)
         if (!(p.dates[i] > p.dates[i - 1])) invalid("product dates must increase strictly");
-    const std::vector<int32_t> contractual = fixing_days(pr);
+    const std::vector<int32_t> contractual = p.graph ? p.graph_tape.days : fixing_days(pr);
     if (contractual.empty()) unsupported("fully fixed payoffs");
     // Observation points (compile/simulation.rs:210-265): the contractual
     // dates and, for a continuous barrier, the dividend ex-times up to the
@@ -742,6 +771,9 @@ void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* p
             d.hist_ext = pr.historical_extremum;
             for (uint32_t i = 0; i < pr.date_count; ++i)
                 if (pr.dates[i] >= 0) p.obs_weight[obs_index(pr.dates[i])] = 1.0;
+            break;
+        case PE_PRODUCT_GRAPH:
+            d.kind = kGraph;
             break;
         case PE_PRODUCT_AMERICAN:
             // exercise dates are the observations (execute_early_exercise needs

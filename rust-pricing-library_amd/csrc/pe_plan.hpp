@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -126,6 +127,24 @@ struct LsmLaunchDeleter {
     void operator()(LsmLaunch* l) const;
 };
 
+// A graph product's tape for the BS batch kernel (compile_graph, pe_graph.cpp).
+struct GraphTape {
+    std::vector<GraphOp> ops;        // scheduled operations
+    std::vector<uint32_t> segs;      // days.size() + 2 boundaries: stage s (-1 = before the path,
+                                     // i = after observation i) is ops [segs[s + 1], segs[s + 2])
+    std::vector<double> consts;      // folded constants (operand kGraphConst | index)
+    std::vector<int32_t> days;       // sorted unique observation days of the tape
+    std::vector<int32_t> pre_days;   // days read before their dividend
+    int32_t last_payment = 0;        // discounting date of the single output
+    int32_t expiry_day = 0;          // last observation (or payment without observations)
+    uint16_t output = 0;             // register or constant operand of the value
+    uint16_t regs = 1;               // registers in use
+};
+
+// Validates and compiles `pr` (kind PE_PRODUCT_GRAPH); `discount(day)` is the
+// discount factor of a payment day.  Throws PlanError.
+void compile_graph(const pe_product& pr, const std::function<double(int32_t)>& discount, GraphTape& g);
+
 struct HostPlan {
     uint64_t id = 0;                 // unique per compiled plan, never reused
     std::unique_ptr<LvLaunch> lv_launch;
@@ -164,6 +183,11 @@ struct HostPlan {
     uint32_t bridge_depth = 0;
 
     ProductDesc product{};
+    bool graph = false;                 // graph product: the tape below is the payoff
+    GraphTape graph_tape;
+    std::vector<pe_graph_node> graph_nodes;
+    std::vector<pe_graph_cashflow> graph_cashflows;
+    std::vector<uint32_t> graph_underlyings;
     std::vector<double> obs_weight;     // per observation (Asian weight or 1 for fixings)
 
     // RQMC
@@ -208,8 +232,8 @@ const uint32_t* direction_words(uint32_t* dimensions);  // embedded Joe--Kuo ass
 constexpr int32_t kNoDay = INT32_MIN;
 
 // Launchers (pe_kernels_f64.hip / pe_kernels_f32.hip).
-hipError_t launch_bs_batch_f64(const BsBatchArgs& a, bool risk, bool cont, hipStream_t s);
-hipError_t launch_bs_batch_f32(const BsBatchArgs& a, bool risk, bool cont, hipStream_t s);
+hipError_t launch_bs_batch_f64(const BsBatchArgs& a, bool risk, bool cont, uint32_t graph_regs, hipStream_t s);
+hipError_t launch_bs_batch_f32(const BsBatchArgs& a, bool risk, bool cont, uint32_t graph_regs, hipStream_t s);
 hipError_t launch_rough_f64(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 grid, uint32_t payoff_items,
                             unsigned long long* error, hipStream_t s);
 hipError_t launch_rough_f32(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 grid, uint32_t payoff_items,

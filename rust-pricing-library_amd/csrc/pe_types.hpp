@@ -13,7 +13,29 @@ constexpr int kMaxBridgeDepth = 64;  // streaming Brownian-bridge stack (LDS)
 constexpr int kBlockThreads = 256;
 constexpr int kComponents = 18;      // the 18-wide pathwise risk vector (PE_COMPONENTS)
 
-enum ProductKind : int32_t { kEuropean = 0, kDigital = 1, kBarrier = 2, kAsian = 3, kLookback = 4 };
+enum ProductKind : int32_t { kEuropean = 0, kDigital = 1, kBarrier = 2, kAsian = 3, kLookback = 4, kGraph = 5 };
+
+// ----------------------------------------------------------- graph payoffs --
+// A graph product's compiled tape (CompiledPayoff, crates/pricing/src/engine/
+// payoff/tape.rs:8-143) as a register program: operations in observation
+// order (an operation runs right after the latest observation it depends
+// on), values in kGraphMaxRegs registers per lane, constants in a table.
+// An operand below kGraphConst is a register, otherwise a constant index.
+constexpr uint32_t kGraphMaxRegs = 32;
+constexpr uint16_t kGraphConst = 0x8000u;
+
+enum GraphCode : uint16_t {
+    kGSpot = 0, kGPreSpot, kGAdd, kGSub, kGMul, kGDiv, kGMin, kGMax, kGInd, kGNeg, kGHit,
+    kGSmoothMin, kGSmoothMax, kGSmoothInd, kGLog, kGExp, kGSelect,
+    kGLess, kGLessEqual, kGEqual, kGNotEqual, kGGreaterEqual, kGGreater
+};
+
+struct GraphOp {
+    double param;            // smoothing half width
+    uint16_t code, out;      // GraphCode, destination register
+    uint16_t a, b, c;        // operands
+    uint16_t pad;
+};
 
 // One product's payoff, specialised per family (the reference builds these as
 // SSA graphs, crates/pricing/src/engine/payoff/compile.rs:35-474).
@@ -159,7 +181,10 @@ struct BsTrade {
     uint32_t resolved;         // continuous barrier: survival fixed by history
     uint32_t expiry_obs;
     uint32_t risk;
-    uint32_t pad;
+    // graph product: ops in [graph_ops + seg[0], ...); seg[i + 1] begins the
+    // ops that run after observation i (n_obs + 2 boundaries from graph_segs)
+    uint32_t graph_ops, graph_segs, graph_consts;
+    uint16_t graph_output, graph_regs;
 };
 
 // A unit range of one trade and scramble, reduced by one workgroup into one
@@ -179,6 +204,9 @@ struct BsBatchArgs {
     const BsItem* items;
     void* partials;            // BlockStat [partial][component]
     unsigned long long* error; // smallest (trade << 40 | unit) whose path failed
+    const GraphOp* graph_ops;  // graph products: tapes, boundaries, constants
+    const uint32_t* graph_segs;
+    const double* graph_consts;
     uint32_t n_items;
     uint32_t components;       // statistics per item (1, 10 or 18)
     uint32_t grid;             // persistent workgroups (0: one per item)

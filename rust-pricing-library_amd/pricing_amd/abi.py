@@ -1,7 +1,7 @@
 """ctypes mirror of include/pe.h (the engine's C ABI)."""
 import ctypes as C
 
-PE_ABI_VERSION = 1
+PE_ABI_VERSION = 2
 
 # pe_status
 PE_OK = 0
@@ -29,6 +29,16 @@ PE_PRODUCT_BARRIER = 2
 PE_PRODUCT_ASIAN = 3
 PE_PRODUCT_LOOKBACK = 4
 PE_PRODUCT_AMERICAN = 5
+PE_PRODUCT_GRAPH = 6
+
+# pe_graph_opcode, by the JSON operation names of product_graph documents
+GRAPH_OPCODES = {
+    "literal": 0, "terminal_spot": 1, "pre_dividend_spot": 2, "add": 3, "subtract": 4, "multiply": 5,
+    "divide": 6, "minimum": 7, "maximum": 8, "indicator": 9, "negate": 10, "barrier_hit_state": 11,
+    "smooth_minimum": 12, "smooth_maximum": 13, "smooth_indicator": 14, "log": 15, "exp": 16,
+    "select": 17, "compare": 18,
+}
+GRAPH_COMPARISONS = {"less": 0, "less_equal": 1, "equal": 2, "not_equal": 3, "greater_equal": 4, "greater": 5}
 
 PE_CALL = 0
 PE_PUT = 1
@@ -86,6 +96,22 @@ class pe_model(C.Structure):
     _fields_ = [("kind", C.c_int), ("volatility", C.c_double), ("lv", pe_local_variance_grid)]
 
 
+class pe_graph_node(C.Structure):
+    _fields_ = [
+        ("id", C.c_uint32),
+        ("opcode", C.c_int),
+        ("operands", C.c_uint32 * 3),
+        ("comparison", C.c_int),
+        ("underlying_id", C.c_uint32),
+        ("observation_day", C.c_int32),
+        ("value", C.c_double),
+    ]
+
+
+class pe_graph_cashflow(C.Structure):
+    _fields_ = [("output_node", C.c_uint32), ("payment_day", C.c_int32)]
+
+
 class pe_product(C.Structure):
     _fields_ = [
         ("kind", C.c_int),
@@ -109,6 +135,12 @@ class pe_product(C.Structure):
         ("dates", _IP),
         ("weights", _DP),
         ("fixings", _DP),
+        ("graph_node_count", C.c_uint32),
+        ("graph_nodes", C.POINTER(pe_graph_node)),
+        ("graph_cashflow_count", C.c_uint32),
+        ("graph_cashflows", C.POINTER(pe_graph_cashflow)),
+        ("graph_underlying_count", C.c_uint32),
+        ("graph_underlyings", C.POINTER(C.c_uint32)),
     ]
 
 

@@ -38,6 +38,7 @@ SOURCES = {
     "pe_plan.cpp": ["-ffp-contract=off"],
     "pe_api.cpp": ["-ffp-contract=off"],
     "pe_surface.cpp": ["-ffp-contract=off"],
+    "pe_graph.cpp": ["-ffp-contract=off"],
 }
 
 

@@ -122,6 +122,69 @@ class Request:
         p.fixings = self._ptr_d([math.nan if f is None else f for f in fixings]) if fixings else None
         return self
 
+    def set_graph_product(self, nodes, cashflows, underlyings):
+        """GraphProduct (crates/pricing/src/product/graph_product.rs:40-137).
+
+        nodes: (id, operation name, operands, extra) with operands the node ids
+        in the order of ``pe_graph_opcode`` and extra a dict holding
+        ``value`` (literal), ``half_width`` (smooth ops), ``operation``
+        (compare), ``underlying_id`` / ``observation_day`` (spots).
+        cashflows: (output node id, payment day); underlyings: declared ids."""
+        p = self.c.product
+        p.kind = abi.PE_PRODUCT_GRAPH
+        arr = (abi.pe_graph_node * len(nodes))()
+        for i, (nid, op, operands, extra) in enumerate(nodes):
+            n = arr[i]
+            n.id = int(nid)
+            if op not in abi.GRAPH_OPCODES:
+                raise UnsupportedRequest(f"graph operation {op}")
+            n.opcode = abi.GRAPH_OPCODES[op]
+            for j, o in enumerate(operands):
+                n.operands[j] = int(o)
+            n.comparison = abi.GRAPH_COMPARISONS.get(extra.get("operation", "less"), 0)
+            n.underlying_id = int(extra.get("underlying_id", 0))
+            n.observation_day = int(extra.get("observation_day", 0))
+            n.value = float(extra.get("value", extra.get("half_width", 0.0)))
+        flows = (abi.pe_graph_cashflow * len(cashflows))()
+        for i, (out, day) in enumerate(cashflows):
+            flows[i].output_node = int(out)
+            flows[i].payment_day = int(day)
+        unders = (C.c_uint32 * len(underlyings))(*[int(u) for u in underlyings])
+        self._keep.extend([arr, flows, unders])
+        p.graph_node_count = len(nodes)
+        p.graph_nodes = C.cast(arr, C.POINTER(abi.pe_graph_node))
+        p.graph_cashflow_count = len(cashflows)
+        p.graph_cashflows = C.cast(flows, C.POINTER(abi.pe_graph_cashflow))
+        p.graph_underlying_count = len(underlyings)
+        p.graph_underlyings = C.cast(unders, C.POINTER(C.c_uint32))
+        return self
+
+    def set_product_graph_document(self, doc, valuation):
+        """A ``product_graph`` document (schemas/product-graph-v1|v2/
+        product_graph.schema.json) as the product; dates become days after
+        `valuation` (a datetime.date)."""
+        if doc.get("document_kind") != "product_graph":
+            raise UnsupportedRequest("not a product_graph document")
+        prod = doc["product"]
+        operand_keys = ("left", "right", "input", "numerator", "denominator", "previous", "hit_weight",
+                        "condition", "then_value", "else_value")
+        nodes = []
+        for n in prod["nodes"]:
+            op = dict(n["operation"])
+            kind = op.pop("type")
+            operands = [op.pop(k) for k in operand_keys if k in op]
+            if kind == "divide":
+                operands = [n["operation"]["numerator"], n["operation"]["denominator"]]
+            elif kind == "barrier_hit_state":
+                operands = [n["operation"]["previous"], n["operation"]["hit_weight"]]
+            elif kind == "select":
+                operands = [n["operation"][k] for k in ("condition", "then_value", "else_value")]
+            if "observation_date" in op:
+                op["observation_day"] = _days(valuation, op.pop("observation_date"))
+            nodes.append((n["id"], kind, operands, op))
+        flows = [(c["output_node"], _days(valuation, c["payment_date"])) for c in prod["cashflows"]]
+        return self.set_graph_product(nodes, flows, prod["underlying_ids"])
+
     def set_pseudo(self, seed, units, antithetic=False, brownian_bridge=False):
         e = self.c.engine
         e.kind = abi.PE_ENGINE_PSEUDO

@@ -12,6 +12,8 @@ from
   fixtures/rough-volatility/two-step-prices.json  (lifted Heston two-step call prices)
   fixtures/replay/early_exercise-windows-x86_64.json  (LSM requests, prices, fitted decision models)
   fixtures/early-exercise/reference-cases-v0.1.json  (decision, basis, scaling, QR cases)
+  fixtures/v3/*.golden.json  (schema-v3 request/result documents)
+  fixtures/product-graph-v1|v2/*.json  (copied to product_graphs/)
 
 The tests read the JSON this script writes; nothing at test time reads the
 reference tree.
@@ -97,6 +99,13 @@ def main():
                  "pricing_result.golden.json", "pricing_result_american.golden.json"):
         with open(os.path.join(ROOT, "fixtures", "v3", name)) as f:
             out["v3_documents"][name] = json.load(f)
+    # product_graph documents (fixtures/product-graph-v1|v2), copied as data
+    os.makedirs(os.path.join(HERE, "product_graphs"), exist_ok=True)
+    for sub, name in (("product-graph-v1", "coupon_and_capped_call.golden.json"), ("product-graph-v2", "digital.json")):
+        with open(os.path.join(ROOT, "fixtures", sub, name)) as f:
+            text = f.read()
+        with open(os.path.join(HERE, "product_graphs", name), "w") as f:
+            f.write(text)
     with open(os.path.join(HERE, "reference_vectors.json"), "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
         f.write("\n")

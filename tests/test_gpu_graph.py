@@ -1,0 +1,112 @@
+"""GPU parity of graph products (the BS batch kernel's register tape)
+against the oracle's lazy graph evaluation: FP64 device mode within 1e-12
+of the oracle (and of the replay fixtures' prices for the graph forms of
+their products), the mixed mode within 0.1 standard errors; the device's
+poison bits raise exactly where the reference's lazy evaluation fails."""
+import copy
+
+import pytest
+
+import _graphs as G
+from _support import oracle_evaluate
+from pricing_amd import abi, engine, request
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-12
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _init():
+    engine.init(0)
+
+
+def _close(a, b, rel=REL):
+    return abs(a - b) <= rel * max(abs(a), abs(b), 1e-300)
+
+
+def _check(req, block=0, mixed=True):
+    ref = oracle_evaluate(req, workers=4, block=block)
+    plan = engine.PricingPlan.compile(req, request.exec_policy(1, block) if block else None)
+    dev = plan.evaluate("fp64")
+    print("graph", ref.estimates[0].value, dev.estimates[0].value, flush=True)
+    assert _close(dev.estimates[0].value, ref.estimates[0].value), (dev.estimates[0].value, ref.estimates[0].value)
+    assert _close(dev.estimates[0].standard_error, ref.estimates[0].standard_error, 1e-10)
+    if mixed:
+        mx = plan.evaluate("mixed")
+        se = ref.estimates[0].standard_error
+        assert abs(mx.estimates[0].value - ref.estimates[0].value) <= 0.1 * se + 1e-12
+    return dev
+
+
+@pytest.mark.parametrize("name", ["pseudo_mc_full_risk", "rqmc_full_risk", "digital_exact_price_only"])
+def test_fixture_products_as_graphs(name):
+    from test_oracle_graph import fixture_graph
+
+    case, doc = G.replay_request(name)
+    g, out, pay = fixture_graph(doc)
+    dev = _check(G.graph_request(doc, g, out, pay), block=case["reduction_block_size"], mixed=False)
+    assert _close(dev.estimates[0].value, case["value"]["value"])
+
+
+@pytest.mark.parametrize("engine_kind", ["pseudo", "rqmc"])
+def test_path_dependent_graphs(engine_kind):
+    _, doc = G.replay_request("pseudo_mc_full_risk" if engine_kind == "pseudo" else "rqmc_full_risk")
+    days = [30 * (i + 1) for i in range(12)]
+    for g, o in (G.asian(days, [1.0 / 12] * 12, 100.0), G.lookback(days, 95.0, call=False),
+                 G.barrier_out(days[-1], days, 100.0, 85.0, down=True)):
+        _check(G.graph_request(doc, g, o, days[-1]))
+
+
+@pytest.mark.parametrize("name", ["digital.json", "coupon_and_capped_call.golden.json"])
+def test_product_graph_documents(name):
+    _, doc = G.replay_request("pseudo_mc_full_risk")
+    req = G.graph_request(doc, G.G(), 0, 0)
+    req.set_product_graph_document(G.golden_graph(name), G.dt.date.fromisoformat(doc["valuation_date"]))
+    _check(req)
+
+
+def test_generated_dags_with_selects():
+    _, doc = G.replay_request("pseudo_mc_full_risk")
+    days = (60, 120, 180)
+    ran = 0
+    for ref, outputs in G.lcg_dags(count=12, ops=20):
+        g, o = G.dag_nodes(ref, outputs[0], days)
+        # every observation feeds the output through the tape, one cash flow
+        _check(G.graph_request(doc, g, o, days[-1]), mixed=False)
+        ran += 1
+    assert ran == 12
+
+
+def test_untaken_log_is_no_error_and_a_live_one_is():
+    _, doc = G.replay_request("pseudo_mc_full_risk")
+    g = G.G()
+    s = g.spot(180)
+    k = g.lit(150.0)
+    c = g.push("compare", (s, k), operation="greater")
+    lg = g.push("log", (g.push("subtract", (s, k)),))
+    out = g.push("select", (c, lg, g.lit(0.0)))
+    _check(G.graph_request(doc, g, out, 180))
+    g2 = G.G()
+    s2 = g2.spot(180)
+    live = g2.push("maximum", (g2.push("log", (g2.push("subtract", (s2, g2.lit(150.0))),)), g2.lit(0.0)))
+    plan = engine.PricingPlan.compile(G.graph_request(doc, g2, live, 180))
+    with pytest.raises(engine.EngineError) as e:
+        plan.evaluate("fp64")
+    assert e.value.status == abi.PE_ERR_NONFINITE_STATE
+    with pytest.raises(RuntimeError):
+        oracle_evaluate(G.graph_request(doc, g2, live, 180))
+
+
+def test_pre_dividend_spot_on_an_ex_date():
+    _, doc = G.replay_request("pseudo_mc_full_risk")
+    doc = copy.deepcopy(doc)
+    doc["market"]["discrete_dividends"] = [{"ex_time": 120 / 365, "quote": {"type": "fixed_cash", "amount": 2.0}}]
+    g = G.G()
+    pre = g.pre_spot(120)
+    post = g.spot(120)
+    term = g.spot(240)
+    drop = g.push("subtract", (pre, post))
+    call = g.push("maximum", (g.push("subtract", (term, g.lit(100.0))), g.lit(0.0)))
+    out = g.push("add", (call, drop))
+    _check(G.graph_request(doc, g, out, 240))
