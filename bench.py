@@ -193,6 +193,32 @@ def batch_block(steps, cpu_seconds, no_cpu, no_fp64):
     return block
 
 
+def graph_block(steps, no_fp64):
+    """Graph-product interpreter: weekly arithmetic Asian calls (52 fixings,
+    2^20 pseudo paths each) as product graphs (register tape in the BS batch
+    kernel) and as the fixed Asian product on the same paths -- one trade
+    (256 workgroups: one wave per SIMD, latency-bound) and a batch of 16
+    trades (4096 workgroups); kernel time per mode."""
+    block = {"workload": "weekly arithmetic Asian call, 52 fixings, 2^20 pseudo paths per trade, BS, price only"}
+    for trades in (1, 16):
+        for mode in ("mixed",) + (() if no_fp64 else ("fp64",)):
+            for name, as_graph in (("graph", True), ("fixed", False)):
+                plans = [engine.PricingPlan.compile(workloads.graph_asian(as_graph=as_graph, seed=0x6EA9_A51A + i))
+                         for i in range(trades)]
+                engine.evaluate_batch(plans, mode)
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    rs = engine.evaluate_batch(plans, mode)
+                wall = (time.perf_counter() - t0) / steps
+                block[f"{name}_{mode}_x{trades}"] = {
+                    "value": sum(r.path_steps for r in rs) / wall, "unit": "path-steps/s",
+                    "kernel_ms": engine.last_kernel_ms()[0], "price": rs[0].estimates[0].value,
+                    "se": rs[0].estimates[0].standard_error}
+            block[f"graph_over_fixed_kernel_{mode}_x{trades}"] = (block[f"graph_{mode}_x{trades}"]["kernel_ms"] /
+                                                                  block[f"fixed_{mode}_x{trades}"]["kernel_ms"])
+    return block
+
+
 def rough_block(steps, cpu_seconds, no_cpu, no_fp64):
     """Config 5 smile: lifted Heston (H=0.1, 20 geometric factors of ratio 2.5),
     21 strikes x 8 maturities priced on one 2^20-path RQMC set over the union
@@ -394,6 +420,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 4096-trade BS/Black-76 batch block")
     ap.add_argument("--no-rough", action="store_true", help="skip the lifted Heston smile block")
+    ap.add_argument("--no-graph", action="store_true", help="skip the graph-product interpreter block")
     ap.add_argument("--no-lsm", action="store_true", help="skip the 256-trade American LSM block")
     ap.add_argument("--no-mlp", action="store_true",
                     help="skip the learned-policy leg of the LSM block (its cooperative launch is not profiled)")
@@ -480,6 +507,8 @@ def main():
         out["batch_bs"] = batch_block(3, args.cpu_seconds, args.no_cpu, args.no_fp64)
     if not args.no_rough:
         out["rough_smile"] = rough_block(5, args.cpu_seconds, args.no_cpu, args.no_fp64)
+    if not args.no_graph:
+        out["graph_payoff"] = graph_block(5, args.no_fp64)
     if not args.no_lsm:
         out["lsm_american"] = lsm_block(args.cpu_seconds, args.no_cpu, args.no_fp64, args.no_mlp)
     print(json.dumps(out))

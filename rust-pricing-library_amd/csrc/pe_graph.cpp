@@ -414,6 +414,9 @@ void compile_graph(const pe_product& pr, const std::function<double(int32_t)>& d
         op.a = in[0];
         op.b = in[1];
         op.c = in[2];
+        double* kv[3] = {&op.ka, &op.kb, &op.kc};
+        for (int j = 0; j < k; ++j)
+            if (in[j] & kGraphConst) *kv[j] = g.consts[in[j] & (kGraphConst - 1u)];
         for (int j = 0; j < k; ++j)
             if (!folded[s.in[j]] && last_use[s.in[j]] == pos) free_mask |= 1u << reg.at(s.in[j]);
         if (free_mask == 0)

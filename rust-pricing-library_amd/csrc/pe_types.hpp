@@ -30,11 +30,13 @@ enum GraphCode : uint16_t {
     kGLess, kGLessEqual, kGEqual, kGNotEqual, kGGreaterEqual, kGGreater
 };
 
-struct GraphOp {
+struct __align__(64) GraphOp {   // one 64-byte scalar load per operation
+    uint32_t code, out;      // GraphCode, destination register
+    uint32_t a, b, c;        // operands (register, or kGraphConst: value in ka/kb/kc)
+    uint32_t pad;
     double param;            // smoothing half width
-    uint16_t code, out;      // GraphCode, destination register
-    uint16_t a, b, c;        // operands
-    uint16_t pad;
+    double ka, kb, kc;       // constant operand values
+    double pad2;
 };
 
 // One product's payoff, specialised per family (the reference builds these as

@@ -28,6 +28,38 @@ def european_bs(units=1 << 16, seed=0x0123456789ABCDEF):
     return req
 
 
+def graph_asian(units=1 << 20, fixings=52, as_graph=True, seed=0x6EA9_A51A):
+    """Weekly arithmetic Asian call (K=100) over one year under Black-Scholes
+    (S0=100, r=5%, sigma=20%), pseudo-MC, price only: as a product graph
+    (the reference's Asian graph, compile.rs:118-160: a running weighted sum,
+    strike, positive part, notional) or as the fixed Asian product."""
+    req = request.Request()
+    dt, df = _flat_curve(0.05)
+    req.set_market(100.0, dt, df, [0.0, 1.0], [1.0, 1.0])
+    req.set_black_scholes(0.2)
+    days = [7 * (i + 1) for i in range(fixings)]
+    w = 1.0 / fixings
+    if as_graph:
+        nodes = [(0, "literal", [], {"value": 0.0})]
+        acc = 0
+        for d in days:
+            n = len(nodes)
+            nodes.append((n, "terminal_spot", [], {"underlying_id": 1, "observation_day": d}))
+            nodes.append((n + 1, "literal", [], {"value": w}))
+            nodes.append((n + 2, "multiply", [n, n + 1], {}))
+            nodes.append((n + 3, "add", [acc, n + 2], {}))
+            acc = n + 3
+        n = len(nodes)
+        nodes += [(n, "literal", [], {"value": 100.0}), (n + 1, "subtract", [acc, n], {}),
+                  (n + 2, "literal", [], {"value": 0.0}), (n + 3, "maximum", [n + 1, n + 2], {}),
+                  (n + 4, "literal", [], {"value": 1.0}), (n + 5, "multiply", [n + 3, n + 4], {})]
+        req.set_graph_product(nodes, [(n + 5, days[-1])], [1])
+    else:
+        req.set_product(abi.PE_PRODUCT_ASIAN, abi.PE_CALL, days[-1], 100.0, dates=days, weights=[w] * fixings)
+    req.set_pseudo(seed, units)
+    return req
+
+
 def weekday_dates(count, start=VALUATION):
     out = []
     d = start
