@@ -374,26 +374,83 @@ void compile_graph(const pe_product& pr, const std::function<double(int32_t)>& d
     std::stable_sort(live_order.begin(), live_order.end(),
                      [&](uint32_t x, uint32_t y) { return stage[x] < stage[y]; });
 
-    // constants and last uses
+    // constants
     g.consts.clear();
     std::map<uint32_t, uint16_t> const_index;
     auto constant = [&](uint32_t id) -> uint16_t {
         auto it = const_index.find(id);
         if (it != const_index.end()) return it->second;
-        if (g.consts.size() >= 0x7fffu) throw PlanError(PE_ERR_SIZE_LIMIT, "graph constants exceed 32767");
+        if (g.consts.size() >= 0x3fffu) throw PlanError(PE_ERR_SIZE_LIMIT, "graph constants exceed 16383");
         const uint16_t k = static_cast<uint16_t>(kGraphConst | g.consts.size());
         g.consts.push_back(folded_value[id]);
         const_index[id] = k;
         return k;
     };
-    std::map<uint32_t, size_t> last_use;
-    for (size_t pos = 0; pos < live_order.size(); ++pos) {
-        const Source& s = nodes[live_order[pos]];
-        for (int j = 0; j < operand_count(s.op); ++j) last_use[s.in[j]] = pos;
+
+    // Fewer dispatched operations (the interpreter's cost is per operation):
+    //  * a Multiply whose only consumer is the Add scheduled right after it
+    //    becomes one a * b + c operation (two roundings in FP64 mode, which
+    //    is built without contraction, so values are unchanged);
+    //  * a spot whose every consumer runs in its own observation's segment is
+    //    read from the observation instead of a register.
+    // A fused product that is non-finite makes the sum non-finite, so the
+    // poison rule is unchanged.
+    std::map<uint32_t, std::vector<uint32_t>> consumers;
+    for (uint32_t id : live_order) {
+        const Source& s = nodes[id];
+        for (int j = 0; j < operand_count(s.op); ++j)
+            if (!folded[s.in[j]]) consumers[s.in[j]].push_back(id);
     }
+    std::map<uint32_t, uint32_t> fused_into;   // Multiply -> its Add
+    for (size_t pos = 0; pos + 1 < live_order.size(); ++pos) {
+        const uint32_t m = live_order[pos], n = live_order[pos + 1];
+        if (nodes[m].op != PE_GRAPH_MULTIPLY || nodes[n].op != PE_GRAPH_ADD || m == output) continue;
+        const auto& cm = consumers[m];
+        if (cm.size() != 1 || cm[0] != n) continue;                 // single use (also not Add(m, m))
+        fused_into[m] = n;
+    }
+    auto emitter = [&](uint32_t id) {   // the operation that evaluates `id`
+        auto it = fused_into.find(id);
+        return it == fused_into.end() ? id : it->second;
+    };
+    std::set<uint32_t> inline_spot;
+    for (uint32_t id : live_order) {
+        const Source& s = nodes[id];
+        if ((s.op != PE_GRAPH_TERMINAL_SPOT && s.op != PE_GRAPH_PRE_DIVIDEND_SPOT) || id == output) continue;
+        bool same = !consumers[id].empty();
+        for (uint32_t c : consumers[id]) same = same && stage[emitter(c)] == stage[id];
+        if (same) inline_spot.insert(id);
+    }
+
+    // emitted operations: (node, code, operand nodes) in schedule order
+    struct Emit {
+        uint32_t id;
+        uint16_t code;
+        int n;
+        uint32_t in[3];
+    };
+    std::vector<Emit> emit;
+    for (uint32_t id : live_order) {
+        if (fused_into.count(id) || inline_spot.count(id)) continue;
+        const Source& s = nodes[id];
+        Emit e{id, code_of(s), operand_count(s.op), {s.in[0], s.in[1], s.in[2]}};
+        if (s.op == PE_GRAPH_ADD) {
+            for (int j = 0; j < 2; ++j) {
+                auto it = fused_into.find(s.in[j]);
+                if (it == fused_into.end() || it->second != id) continue;
+                const Source& m = nodes[s.in[j]];
+                e = Emit{id, kGMulAdd, 3, {m.in[0], m.in[1], s.in[1 - j]}};
+                break;
+            }
+        }
+        emit.push_back(e);
+    }
+    std::map<uint32_t, size_t> last_use;
+    for (size_t pos = 0; pos < emit.size(); ++pos)
+        for (int j = 0; j < emit[pos].n; ++j) last_use[emit[pos].in[j]] = pos;
     last_use[output] = SIZE_MAX;
 
-    // linear-scan register assignment over the schedule
+    // linear-scan register assignment over the emitted schedule
     std::map<uint32_t, uint16_t> reg;
     uint32_t free_mask = (kGraphMaxRegs == 32) ? 0xffffffffu : ((1u << kGraphMaxRegs) - 1u);
     uint32_t used = 0;
@@ -401,24 +458,29 @@ void compile_graph(const pe_product& pr, const std::function<double(int32_t)>& d
     g.segs.assign(g.days.size() + 2, 0u);
     int current_stage = -1;
     g.segs[0] = 0;
-    for (size_t pos = 0; pos < live_order.size(); ++pos) {
-        const uint32_t id = live_order[pos];
-        const Source& s = nodes[id];
-        while (current_stage < stage[id]) g.segs[static_cast<size_t>(++current_stage) + 1] = static_cast<uint32_t>(g.ops.size());
+    auto operand = [&](uint32_t node) -> uint16_t {
+        if (folded[node]) return constant(node);
+        if (inline_spot.count(node))
+            return nodes[node].op == PE_GRAPH_TERMINAL_SPOT ? kGraphPost : kGraphPre;
+        return reg.at(node);
+    };
+    for (size_t pos = 0; pos < emit.size(); ++pos) {
+        const Emit& e = emit[pos];
+        const Source& s = nodes[e.id];
+        while (current_stage < stage[e.id]) g.segs[static_cast<size_t>(++current_stage) + 1] = static_cast<uint32_t>(g.ops.size());
         GraphOp op{};
-        op.code = code_of(s);
+        op.code = e.code;
         op.param = s.value;
         uint16_t in[3] = {0, 0, 0};
-        const int k = operand_count(s.op);
-        for (int j = 0; j < k; ++j) in[j] = folded[s.in[j]] ? constant(s.in[j]) : reg.at(s.in[j]);
+        for (int j = 0; j < e.n; ++j) in[j] = operand(e.in[j]);
         op.a = in[0];
         op.b = in[1];
         op.c = in[2];
         double* kv[3] = {&op.ka, &op.kb, &op.kc};
-        for (int j = 0; j < k; ++j)
+        for (int j = 0; j < e.n; ++j)
             if (in[j] & kGraphConst) *kv[j] = g.consts[in[j] & (kGraphConst - 1u)];
-        for (int j = 0; j < k; ++j)
-            if (!folded[s.in[j]] && last_use[s.in[j]] == pos) free_mask |= 1u << reg.at(s.in[j]);
+        for (int j = 0; j < e.n; ++j)
+            if (in[j] < kGraphMaxRegs && last_use[e.in[j]] == pos) free_mask |= 1u << in[j];
         if (free_mask == 0)
             throw PlanError(PE_ERR_SIZE_LIMIT, "graph needs more than " + std::to_string(kGraphMaxRegs) +
                                                    " live values at once");
@@ -426,7 +488,7 @@ void compile_graph(const pe_product& pr, const std::function<double(int32_t)>& d
         free_mask &= ~(1u << r);
         used = std::max<uint32_t>(used, r + 1u);
         op.out = r;
-        reg[id] = r;
+        reg[e.id] = r;
         g.ops.push_back(op);
     }
     while (current_stage < static_cast<int>(g.days.size())) g.segs[static_cast<size_t>(++current_stage) + 1] = static_cast<uint32_t>(g.ops.size());

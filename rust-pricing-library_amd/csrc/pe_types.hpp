@@ -23,11 +23,16 @@ enum ProductKind : int32_t { kEuropean = 0, kDigital = 1, kBarrier = 2, kAsian =
 // An operand below kGraphConst is a register, otherwise a constant index.
 constexpr uint32_t kGraphMaxRegs = 32;
 constexpr uint16_t kGraphConst = 0x8000u;
+// operands read straight from the current observation (a spot node whose
+// every consumer runs in its own observation's segment is not materialised)
+constexpr uint16_t kGraphPost = 0x4000u;
+constexpr uint16_t kGraphPre = 0x4001u;
 
 enum GraphCode : uint16_t {
     kGSpot = 0, kGPreSpot, kGAdd, kGSub, kGMul, kGDiv, kGMin, kGMax, kGInd, kGNeg, kGHit,
     kGSmoothMin, kGSmoothMax, kGSmoothInd, kGLog, kGExp, kGSelect,
-    kGLess, kGLessEqual, kGEqual, kGNotEqual, kGGreaterEqual, kGGreater
+    kGLess, kGLessEqual, kGEqual, kGNotEqual, kGGreaterEqual, kGGreater,
+    kGMulAdd   // a * b + c: a Multiply fused into the Add that is its only consumer
 };
 
 struct __align__(64) GraphOp {   // one 64-byte scalar load per operation
