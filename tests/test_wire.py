@@ -75,3 +75,14 @@ def test_result_document_has_the_schema_fields():
     assert set(golden_doc) <= set(doc) | {"monte_carlo"}
     assert set(doc) <= {"document_kind", "schema_version", "value", "risks", "diagnostics", "monte_carlo", "replay"}
     assert set(doc["replay"]) == set(golden_doc["replay"])
+
+
+def test_v3_golden_american_documents_pair_up():
+    # fixtures/v3: the American result document records the fingerprint of
+    # the American request document (the European result holds a zero
+    # placeholder fingerprint, so only the American pair pins it)
+    v = golden()["v3_documents"]
+    fp = wire.request_fingerprint(v["pricing_request_american.golden.json"])
+    replay = v["pricing_result_american.golden.json"]["replay"]
+    assert fp == replay["request_fingerprint"]
+    assert fp == replay["migration"]["pre_migration_fingerprint"] == replay["migration"]["post_migration_fingerprint"]
