@@ -1,4 +1,4 @@
-"""Named workloads (BASELINE configurations) built as engine requests.
+"""Named workloads (the benchmark configurations: local-vol path-dependent set, trade batch, American batch, lifted-Heston smile, graph Asian) built as engine requests.
 
 Each builder returns Request objects; sizes are parameters so tests can run
 the same workload shape at reduced path counts.
