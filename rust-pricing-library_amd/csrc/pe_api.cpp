@@ -535,7 +535,7 @@ void prepare_rough(std::vector<HostPlan*>& plans, pe_backend backend) {
     };
     std::vector<RoughStep> steps(n);
     std::vector<double> den(static_cast<size_t>(n) * nf);
-    std::vector<float> rden(den.size());
+    std::vector<double> rden(den.size());
     for (uint32_t j = 0; j < n; ++j) {
         const double dt = nodes[j + 1] - nodes[j];
         steps[j] = RoughStep{dt, std::sqrt(dt), scale_at(nodes[j + 1]), 0.0};
@@ -543,7 +543,7 @@ void prepare_rough(std::vector<HostPlan*>& plans, pe_backend backend) {
             const double d = 1.0 + h.rates[i] * dt;
             if (!std::isfinite(d)) throw PlanError(PE_ERR_INVALID_ARGUMENT, "lifted time scale overflow");
             den[static_cast<size_t>(j) * nf + i] = d;
-            rden[static_cast<size_t>(j) * nf + i] = static_cast<float>(1.0 / d);
+            rden[static_cast<size_t>(j) * nf + i] = 1.0 / d;
         }
     }
     // store slots: the distinct expiry nodes of the plans
@@ -578,7 +578,7 @@ void prepare_rough(std::vector<HostPlan*>& plans, pe_backend backend) {
     RoughArgs a{};
     a.steps = d_steps.as<RoughStep>();
     a.denominators = d_den.as<double>();
-    a.reciprocals = d_rden.as<float>();
+    a.reciprocals = d_rden.as<double>();
     a.sobol = d_sobol.as<uint32_t>();
     a.store_nodes = d_store_nodes.as<uint32_t>();
     a.store = d_store.ptr;
@@ -593,11 +593,7 @@ void prepare_rough(std::vector<HostPlan*>& plans, pe_backend backend) {
     a.nu = h.nu;
     a.rho = h.rho;
     a.rho_bar = std::sqrt(std::max(1.0 - h.rho * h.rho, 0.0));
-    for (uint32_t i = 0; i < nf; ++i) {
-        a.weights[i] = h.weights[i];
-        a.weights32[i] = static_cast<float>(h.weights[i]);
-    }
-    a.v0_32 = static_cast<float>(h.v0);
+    for (uint32_t i = 0; i < nf; ++i) a.weights[i] = h.weights[i];
     a.n_steps = n;
     a.n_factors = nf;
     a.n_store = static_cast<uint32_t>(store_nodes.size());

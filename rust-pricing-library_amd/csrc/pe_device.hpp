@@ -86,6 +86,37 @@ PE_HD double normal_from_word_f64(uint32_t word) {
     return centered < 0.0 ? -m : m;
 }
 
+// The same FP64 arithmetic for kernels with many other live values: every
+// coefficient passes through an empty asm statement, so it is materialised
+// in scalar registers at its use instead of being hoisted out of the
+// kernel's loops into 50-odd vector-register pairs.  Bit-identical results.
+__device__ __forceinline__ double pe_sgpr(double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+s"(c));
+#endif
+    return c;
+}
+
+__device__ __forceinline__ double normal_from_word_f64_lean(uint32_t word) {
+    const double p = (static_cast<double>(word) + 0.5) * (1.0 / 4294967296.0);
+    const double centered = p - 0.5;
+    if (fabs(centered) <= 0.425) {
+        const double a = pe_sgpr(0.180625) - centered * centered;
+        return centered * PE_AS241_A(pe_sgpr, a) / PE_AS241_B(pe_sgpr, a);
+    }
+    const double tail = centered < 0.0 ? p : 1.0 - p;
+    const double r = sqrt(-log(tail));
+    double m;
+    if (r <= 5.0) {
+        const double a = r - 1.6;
+        m = PE_AS241_C(pe_sgpr, a) / PE_AS241_D(pe_sgpr, a);
+    } else {
+        const double a = r - 5.0;
+        m = PE_AS241_E(pe_sgpr, a) / PE_AS241_F(pe_sgpr, a);
+    }
+    return centered < 0.0 ? -m : m;
+}
+
 // FP32: the same rational approximations evaluated in FP32.  The centred
 // coordinate and the tail probability come from the integer word directly,
 // so FP32 rounding never maps a word to 0, 1 or the wrong side of 1/2:

@@ -232,7 +232,7 @@ struct RoughStep {       // per time step of the shared grid
 struct RoughArgs {
     const RoughStep* steps;
     const double* denominators;   // [step][factor]: 1 + x_i dt (FP64 mode divides by it)
-    const float* reciprocals;     // [step][factor]: 1 / (1 + x_i dt) (mixed mode)
+    const double* reciprocals;    // [step][factor]: 1 / (1 + x_i dt) (mixed mode multiplies)
     const uint32_t* sobol;        // [scramble][dim][33], dims = 2 * steps
     const uint32_t* store_nodes;  // ascending grid nodes whose spot is stored
     void* store;                  // [slot][path]: double (FP64 mode) or float
@@ -240,8 +240,6 @@ struct RoughArgs {
     double spot, scale0;
     double v0, kappa, theta, nu, rho, rho_bar;
     double weights[kMaxRoughFactors];
-    float weights32[kMaxRoughFactors];
-    float v0_32;
     uint32_t n_steps, n_factors, n_store, rqmc, antithetic;
 };
 

@@ -118,9 +118,15 @@ def _dividend_days():
 @pytest.mark.parametrize("name", sorted(PRODUCTS))
 @pytest.mark.parametrize("smoothing", [2.0, None])
 def test_dividend_products_fp64_match_oracle(name, smoothing):
-    if smoothing is None and name in ("digital", "barrier_jump"):
-        pytest.skip("pathwise risk of a discontinuous payoff needs smoothing")
     req = _req(PRODUCTS[name], smoothing=smoothing, divs=_dividend_days())
+    if smoothing is None and name in ("digital", "barrier_jump"):
+        # pathwise risk of a discontinuous payoff needs compact-C2 smoothing:
+        # the plan compiler rejects the request
+        with pytest.raises(engine.EngineError) as err:
+            engine.PricingPlan.compile(req)
+        assert err.value.status == abi.PE_ERR_INVALID_ARGUMENT
+        assert "smoothing" in str(err.value)
+        return
     gpu = engine.PricingPlan.compile(req).evaluate("fp64")
     ref = oracle_evaluate(req, 8)
     comps = list(range(10)) + (DIAG if name.startswith("cont") else [])
