@@ -342,6 +342,24 @@ void pe_plan_free(pe_plan* plan);
 /* Mirrors PricingPlan::evaluate (crates/pricing/src/engine/risk/pricing.rs:86-88). */
 pe_status pe_plan_evaluate(const pe_plan* plan, pe_backend backend, pe_result* out);
 
+/* Allocation-free variant of pe_plan_evaluate for Black-Scholes / Black-76
+ * plans (price, Greeks, continuous barriers, graph products; RQMC or
+ * pseudo-MC).  No reference counterpart: it is for callers that cannot
+ * allocate device memory -- code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  Every device buffer of the
+ * evaluation (trade record, observation and Sobol tables, graph tape, work
+ * items, block partials, error word) lives in one fixed 8 MiB workspace per
+ * GPU, a file-scope device array held until the process ends; a plan that
+ * needs more returns PE_ERR_SIZE_LIMIT (roughly: 4096-unit chunks x
+ * scrambles x 40 B per statistic, plus the Sobol table of 132 B per
+ * dimension and scramble).  The call makes no hipMalloc/hipFree, pinned-host
+ * allocation or event creation; host-side bookkeeping uses the C++ heap.
+ * All calls on one GPU share its workspace: they are serialised inside the
+ * library, and the workspace keeps the last call's contents.  Local-vol,
+ * lifted Heston and American plans return PE_ERR_UNSUPPORTED.  device_ms
+ * is 0 (no timing events).  Results equal pe_plan_evaluate's. */
+pe_status pe_plan_evaluate_noalloc(const pe_plan* plan, pe_backend backend, pe_result* out);
+
 /* Evaluates several plans that share one simulated path set (same market,
  * model, engine and simulation grid; e.g. a barrier and an Asian on the same
  * local-vol paths) in one launch.  Each result equals what pe_plan_evaluate

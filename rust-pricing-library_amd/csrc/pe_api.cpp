@@ -102,6 +102,7 @@ void ensure_init() {
     if (count <= 0) throw PlanError(PE_ERR_DEVICE, "no HIP device");
     check(hipSetDevice(g_device), "hipSetDevice");
     check(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking), "hipStreamCreate");
+    check(scratch_setup(g_device), "workspace symbol address");
     g_initialised = true;
 }
 
@@ -439,6 +440,119 @@ void evaluate_bs_batch(std::vector<HostPlan*>& plans, pe_backend backend, pe_res
     }
     g_kernel_ms = ms;
     g_kernel_launches = launches;
+}
+
+// ------------------------------------------------- allocation-free path --
+// One BS/Black-76 plan through the batch kernel with every device buffer --
+// trade record, observation table, Sobol table, graph tape, work items,
+// block partials and the error word -- carved out of this device's fixed
+// workspace (pe_noalloc.hip).  No device or pinned-host allocation, no event
+// creation; the workspace is shared by every call on the device, so calls
+// are serialised by g_noalloc_mutex and each call rewrites what it reads.
+std::mutex g_noalloc_mutex;
+
+void evaluate_bs_noalloc(HostPlan& p, pe_backend backend, pe_result* out) {
+    if (p.lv || p.rough || p.american)
+        throw PlanError(PE_ERR_UNSUPPORTED, "the allocation-free entry point prices Black-Scholes / Black-76 plans");
+    unsigned char* base = static_cast<unsigned char*>(scratch_pointer(g_device));
+    if (!base) throw PlanError(PE_ERR_DEVICE, "workspace address not set up (pe_init)");
+    const pe_engine& e = p.req.engine;
+    const bool rqmc = e.kind == PE_ENGINE_RQMC;
+    const uint32_t scrambles = rqmc ? e.scrambles : 1u;
+    const int g = p.graph ? 4 : (p.risk ? 1 : 0) | (p.continuous ? 2 : 0);
+    const bool risk = g < 4 && (g & 1) != 0, cont = g < 4 && (g & 2) != 0;
+    const uint32_t comps = cont ? PE_COMPONENTS : (risk ? 10u : 1u);
+    const uint64_t chunks64 = (e.units + kItemUnits - 1) / kItemUnits;
+    const uint64_t n_items64 = chunks64 * scrambles;
+    if (rqmc && p.host_sobol.empty()) p.host_sobol = build_sobol_table(e.seed, e.scrambles, p.sobol_dims);
+    // layout: error word | trade | observations | Sobol | graph ops | segs | consts | items | partials
+    auto align = [](size_t x) { return (x + 255u) & ~static_cast<size_t>(255u); };
+    size_t off = 0;
+    const size_t o_err = off; off = align(off + sizeof(unsigned long long));
+    const size_t o_trade = off; off = align(off + sizeof(BsTrade));
+    const size_t o_obs = off; off = align(off + p.bs_obs.size() * sizeof(BsObs));
+    const size_t o_sobol = off; off = align(off + p.host_sobol.size() * sizeof(uint32_t));
+    const GraphTape& gt = p.graph_tape;
+    const size_t o_gops = off; off = align(off + (p.graph ? gt.ops.size() * sizeof(GraphOp) : 0));
+    const size_t o_gsegs = off; off = align(off + (p.graph ? gt.segs.size() * sizeof(uint32_t) : 0));
+    const size_t o_gconsts = off; off = align(off + (p.graph ? gt.consts.size() * sizeof(double) : 0));
+    const size_t o_items = off;
+    const size_t need = o_items + align(n_items64 * sizeof(BsItem)) + n_items64 * comps * sizeof(BlockStat);
+    if (n_items64 > 0x7fffffffull || need > scratch_bytes())
+        throw PlanError(PE_ERR_SIZE_LIMIT, "the plan needs " + std::to_string(need) +
+                                               " bytes of device workspace; the allocation-free entry point has " +
+                                               std::to_string(scratch_bytes()));
+    const uint32_t chunks = static_cast<uint32_t>(chunks64), n_items = static_cast<uint32_t>(n_items64);
+    off = align(o_items + n_items64 * sizeof(BsItem));
+    const size_t o_partials = off;
+    BsTrade t = p.trade;
+    t.obs_offset = 0;
+    t.sobol_offset = 0;
+    if (p.graph) {
+        t.graph_ops = t.graph_segs = t.graph_consts = 0;
+        if (gt.segs.size() != p.bs_obs.size() + 2) throw PlanError(PE_ERR_INVALID_ARGUMENT, "graph schedule");
+    }
+    // items in unit order (one workgroup per item, partial = item index)
+    std::vector<BsItem> items(n_items);
+    for (uint32_t sc = 0; sc < scrambles; ++sc)
+        for (uint32_t c = 0; c < chunks; ++c) {
+            BsItem& it = items[static_cast<size_t>(sc) * chunks + c];
+            it = BsItem{};
+            it.scramble = sc;
+            it.unit_begin = static_cast<uint64_t>(c) * kItemUnits;
+            it.unit_end = std::min<uint64_t>(it.unit_begin + kItemUnits, e.units);
+            it.partial = sc * chunks + c;
+        }
+    auto put = [&](size_t at, const void* src, size_t bytes) {
+        if (bytes) check(hipMemcpyAsync(base + at, src, bytes, hipMemcpyHostToDevice, g_stream), "workspace upload");
+    };
+    const unsigned long long none = ~0ull;
+    put(o_err, &none, sizeof(none));
+    put(o_trade, &t, sizeof(t));
+    put(o_obs, p.bs_obs.data(), p.bs_obs.size() * sizeof(BsObs));
+    put(o_sobol, p.host_sobol.data(), p.host_sobol.size() * sizeof(uint32_t));
+    if (p.graph) {
+        put(o_gops, gt.ops.data(), gt.ops.size() * sizeof(GraphOp));
+        put(o_gsegs, gt.segs.data(), gt.segs.size() * sizeof(uint32_t));
+        put(o_gconsts, gt.consts.data(), gt.consts.size() * sizeof(double));
+    }
+    put(o_items, items.data(), items.size() * sizeof(BsItem));
+    BsBatchArgs a{};
+    a.trades = reinterpret_cast<const BsTrade*>(base + o_trade);
+    a.obs = reinterpret_cast<const BsObs*>(base + o_obs);
+    a.sobol = reinterpret_cast<const uint32_t*>(base + o_sobol);
+    a.items = reinterpret_cast<const BsItem*>(base + o_items);
+    a.partials = base + o_partials;
+    a.error = reinterpret_cast<unsigned long long*>(base + o_err);
+    a.graph_ops = reinterpret_cast<const GraphOp*>(base + o_gops);
+    a.graph_segs = reinterpret_cast<const uint32_t*>(base + o_gsegs);
+    a.graph_consts = reinterpret_cast<const double*>(base + o_gconsts);
+    a.n_items = n_items;
+    a.components = comps;
+    a.grid = 0;
+    const uint32_t graph_regs = p.graph ? gt.regs : 0u;
+    check(backend == PE_BACKEND_GPU_FP64 ? launch_bs_batch_f64(a, risk, cont, graph_regs, g_stream)
+                                         : launch_bs_batch_f32(a, risk, cont, graph_regs, g_stream),
+          "bs kernel launch");
+    std::vector<BlockStat> partials(static_cast<size_t>(n_items) * comps);
+    unsigned long long err = none;
+    check(hipMemcpyAsync(partials.data(), base + o_partials, partials.size() * sizeof(BlockStat),
+                         hipMemcpyDeviceToHost, g_stream),
+          "partials copy");
+    check(hipMemcpyAsync(&err, base + o_err, sizeof(err), hipMemcpyDeviceToHost, g_stream), "error copy");
+    check(hipStreamSynchronize(g_stream), "stream synchronize");
+    if (err != none) {
+        const uint64_t unit = err & ((1ull << 40) - 1);
+        out->failing_unit = unit;
+        throw PlanError(PE_ERR_NONFINITE_STATE,
+                        "sampling unit " + std::to_string(unit) +
+                            ((err >> 63) ? ": non-finite graph payoff value"
+                                         : ": invalid barrier bridge state (non-positive transformed barrier or state)"));
+    }
+    std::memset(out, 0, sizeof(*out));
+    assemble(p, partials, chunks, scrambles, comps, 0, comps, out);
+    g_kernel_ms = 0.0;
+    g_kernel_launches = 1;
 }
 
 // ---------------------------------------------------------- lifted Heston --
@@ -1432,6 +1546,16 @@ pe_status pe_plan_evaluate(const pe_plan* plan, pe_backend backend, pe_result* o
             std::vector<HostPlan*> plans{&p};
             evaluate_bs_batch(plans, backend, out);
         }
+    });
+}
+
+pe_status pe_plan_evaluate_noalloc(const pe_plan* plan, pe_backend backend, pe_result* out) {
+    if (!plan || !out) return fail(PE_ERR_INVALID_ARGUMENT, "null argument");
+    if (backend != PE_BACKEND_GPU_FP64 && backend != PE_BACKEND_GPU_MIXED) return fail(PE_ERR_INVALID_ARGUMENT, "backend");
+    return guarded([&] {
+        ensure_init();
+        std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+        evaluate_bs_noalloc(const_cast<pe_plan*>(plan)->host, backend, out);
     });
 }
 

@@ -192,6 +192,7 @@ struct HostPlan {
 
     // RQMC
     uint32_t sobol_dims = 0;
+    std::vector<uint32_t> host_sobol;   // scrambled direction table, built on first use (allocation-free path)
 
     // rough-volatility adapter (lifted Heston); the carrier's BS fields above
     // supply the payoff, discount and observation times
@@ -230,6 +231,11 @@ void compile_bridge_program(const std::vector<double>& times, std::vector<Bridge
 const uint32_t* direction_words(uint32_t* dimensions);  // embedded Joe--Kuo asset (host order)
 
 constexpr int32_t kNoDay = INT32_MIN;
+
+// Fixed device workspace of the allocation-free entry point (pe_noalloc.hip).
+size_t scratch_bytes();
+hipError_t scratch_setup(int device);   // with `device` current; once per device
+void* scratch_pointer(int device);
 
 // Launchers (pe_kernels_f64.hip / pe_kernels_f32.hip).
 hipError_t launch_bs_batch_f64(const BsBatchArgs& a, bool risk, bool cont, uint32_t graph_regs, hipStream_t s);

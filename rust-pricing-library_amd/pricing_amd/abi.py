@@ -326,6 +326,7 @@ PE_SYMBOLS = {
     "pe_plan_compile": (C.c_int, [C.POINTER(pe_request), C.POINTER(pe_exec_policy), C.POINTER(C.c_void_p)]),
     "pe_plan_free": (None, [C.c_void_p]),
     "pe_plan_evaluate": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(pe_result)]),
+    "pe_plan_evaluate_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(pe_result)]),
     "pe_plans_evaluate_shared": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.c_int, C.POINTER(pe_result)]),
     "pe_batch_evaluate": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.c_int, C.POINTER(pe_result)]),
     "pe_lifted_heston_from_rough": (C.c_int, [C.c_double, C.c_uint32, C.c_double, _DP, _DP]),

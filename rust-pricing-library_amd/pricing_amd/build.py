@@ -35,6 +35,7 @@ SOURCES = {
     "pe_kernels_f64.hip": ["-ffp-contract=off"],
     "pe_kernels_f32.hip": ["-ffp-contract=fast"],
     "pe_dupire.hip": ["-ffp-contract=off"],
+    "pe_noalloc.hip": ["-ffp-contract=off"],
     "pe_plan.cpp": ["-ffp-contract=off"],
     "pe_api.cpp": ["-ffp-contract=off"],
     "pe_surface.cpp": ["-ffp-contract=off"],

@@ -72,6 +72,13 @@ class PricingPlan:
         _check(load().pe_plan_evaluate(self._handle, BACKENDS[backend], C.byref(res)))
         return res
 
+    def evaluate_noalloc(self, backend="fp64"):
+        """pe_plan_evaluate_noalloc: the same result with every device buffer
+        in the library's fixed per-GPU workspace (BS / Black-76 plans)."""
+        res = abi.pe_result()
+        _check(load().pe_plan_evaluate_noalloc(self._handle, BACKENDS[backend], C.byref(res)))
+        return res
+
     def free(self):
         if self._handle:
             load().pe_plan_free(self._handle)
