@@ -16,6 +16,12 @@
  * Dates are whole days counted from the valuation date (ACT/365F year fraction
  * = days / 365, crates/pricing/src/core/date.rs:223-231).
  *
+ * Threading: like the reference's PricingPlan::evaluate (&self, Send + Sync),
+ * distinct plans may be evaluated concurrently from several host threads.
+ * Each evaluate call runs on a HIP stream of its own and holds the locks of
+ * the plans it uses, so concurrent calls on the same plan serialise.  A plan
+ * must not be freed while a call uses it.
+ *
  * The oracle library (oracle/liboracle.so) accepts the same request structs
  * through oracle_evaluate(); it is test infrastructure, not part of this ABI.
  */
@@ -29,7 +35,7 @@
 extern "C" {
 #endif
 
-#define PE_ABI_VERSION 2
+#define PE_ABI_VERSION 3
 
 /* Error taxonomy mirrors MonteCarloError (crates/pricing/src/api/error.rs:256-300)
  * at the granularity a C caller can act on. */
@@ -238,6 +244,11 @@ typedef struct pe_risk {
     double gamma_bump;
     int32_t smoothing;          /* 1: PayoffSmoothing::CompactC2 */
     double smoothing_half_width;
+    /* RiskRequest aad_tile_capacity / checkpoint_interval overrides (0: the
+     * reference's defaults, min(256, block size) and 64).  The device has no
+     * AAD tape; they are validated and echoed in pe_mc_diagnostics. */
+    uint32_t aad_tile_capacity;
+    uint32_t checkpoint_interval;
 } pe_risk;
 
 /* Continuation-value regression of an early-exercise product. */
@@ -307,6 +318,44 @@ enum {
     PE_COMPONENTS = 18
 };
 
+/* EstimatorKind, CurveRegion (crates/pricing/src/market/curve.rs:18-22) and
+ * PayoffValuationKind (crates/pricing/src/api/mc_result.rs:75-78). */
+typedef enum pe_estimator { PE_ESTIMATOR_PSEUDO_MONTE_CARLO = 0, PE_ESTIMATOR_RQMC = 1 } pe_estimator;
+typedef enum pe_curve_region {
+    PE_CURVE_PILLAR = 0, PE_CURVE_INTERPOLATED = 1, PE_CURVE_RIGHT_EXTRAPOLATED = 2
+} pe_curve_region;
+typedef enum pe_valuation_kind {
+    PE_VALUATION_EXACT_CONTRACTUAL = 0, PE_VALUATION_SMOOTHED_SURROGATE = 1
+} pe_valuation_kind;
+
+/* MonteCarloDiagnostics (crates/pricing/src/api/mc_result.rs:226-247) except
+ * the payoff-graph fingerprint and the optional smoothing / path-state /
+ * bridge sub-blocks (the bridge diagnostics are result components 10-17).
+ * RQMC plans carry the BLAKE3-256 direction checksum (of the Joe-Kuo asset)
+ * and scramble checksum (of every scramble's digital shift and matrix rows,
+ * big-endian, scramble-major; crates/pricing/src/engine/sampling/qmc.rs:236-259).
+ * policy_version, aad_tile_policy_version and checkpoint_policy_version are
+ * the reference's policy versions (1). */
+typedef struct pe_mc_diagnostics {
+    uint64_t master_seed;
+    pe_estimator estimator;
+    uint32_t scramble_count;            /* RQMC; 0 = none (pseudo-MC) */
+    int32_t has_checksums;              /* 1 for RQMC */
+    uint8_t direction_checksum[32];
+    uint8_t scramble_checksum[32];
+    uint32_t policy_version;
+    uint32_t worker_threads;
+    uint64_t reduction_block_size;
+    uint32_t aad_tile_policy_version;
+    uint32_t aad_tile_capacity;
+    uint32_t checkpoint_policy_version;
+    uint32_t checkpoint_interval;
+    int32_t antithetic;
+    pe_curve_region discount_region;    /* discount curve at the payment time */
+    pe_curve_region dividend_region;    /* dividend curve at the expiry (0 without observations) */
+    pe_valuation_kind valuation_kind;
+} pe_mc_diagnostics;
+
 typedef struct pe_result {
     /* per component, over independent units (pseudo) or scramble means (RQMC) */
     pe_estimate estimates[PE_COMPONENTS];
@@ -322,6 +371,7 @@ typedef struct pe_result {
     uint32_t sobol_dimension;                 /* RQMC effective dimension */
     double device_ms;                         /* kernel time, HIP events */
     uint64_t failing_unit;                    /* first failing sampling unit */
+    pe_mc_diagnostics diagnostics;            /* of the valuation engine */
 } pe_result;
 
 /* ------------------------------------------------------------ entry points -- */
@@ -338,6 +388,14 @@ int32_t pe_abi_version(void);
 pe_status pe_plan_compile(const pe_request* request, const pe_exec_policy* policy,
                           pe_plan** out);
 void pe_plan_free(pe_plan* plan);
+
+/* The MonteCarloDiagnostics a plan's evaluation reports, known at compile
+ * time (host only; every evaluate copies it into pe_result.diagnostics). */
+pe_status pe_plan_diagnostics(const pe_plan* plan, pe_mc_diagnostics* out);
+
+/* BLAKE3-256 of `bytes` bytes (the checksum primitive above; exposed for
+ * verification, no reference counterpart). */
+pe_status pe_blake3_256(const void* data, size_t bytes, uint8_t out[32]);
 
 /* Mirrors PricingPlan::evaluate (crates/pricing/src/engine/risk/pricing.rs:86-88). */
 pe_status pe_plan_evaluate(const pe_plan* plan, pe_backend backend, pe_result* out);
@@ -412,6 +470,10 @@ typedef struct pe_lsm_decision {
  * crates/pricing/src/engine/risk/early_exercise.rs:24-300): the in-sample
  * value of the trained policy, valuation exercise counts per date and the
  * decision model per non-terminal date; plus device time per phase. */
+typedef enum pe_random_domain {     /* RandomDomain (crates/pricing/src/engine/sampling/random.rs:13-20) */
+    PE_DOMAIN_VALUATION = 0, PE_DOMAIN_LSM_TRAIN = 1, PE_DOMAIN_RQMC_SCRAMBLE = 2
+} pe_random_domain;
+
 typedef struct pe_lsm_report {
     double in_sample_value;
     double training_ms;             /* training paths + backward induction, HIP events */
@@ -421,6 +483,23 @@ typedef struct pe_lsm_report {
     uint64_t* exercise_counts;      /* [capacity] or NULL */
     pe_lsm_decision* decisions;     /* [capacity] or NULL */
     double path_ms;                 /* training-path generation, part of training_ms */
+    /* EarlyExerciseDiagnostics (crates/pricing/src/api/mc_result.rs:249-273),
+     * all but the policy fingerprint and the regression warnings. */
+    double* exercise_probabilities; /* [capacity] or NULL: counts / valuation trajectories */
+    int32_t* dividend_collisions;   /* [capacity] or NULL: an ex-dividend time equals the date's */
+    uint32_t* stopping_indices;     /* [stopping_capacity] or NULL: exercise-date index per valuation
+                                       trajectory, (scramble, unit, antithetic pass) order */
+    uint64_t stopping_capacity;
+    pe_random_domain training_random_domain;
+    pe_random_domain valuation_random_domain;
+    uint64_t training_sampling_units, training_trajectories;
+    uint64_t valuation_sampling_units, valuation_trajectories;
+    int32_t training_has_checksums, valuation_has_checksums;
+    uint8_t training_direction_checksum[32], training_scramble_checksum[32];
+    uint8_t valuation_direction_checksum[32], valuation_scramble_checksum[32];
+    uint32_t policy_feature_count, policy_max_degree;   /* PolynomialBasisSpec */
+    double itm_abs_tolerance, abs_rank_tolerance, rel_rank_tolerance;
+    uint64_t max_matrix_elements;
 } pe_lsm_report;
 
 /* Prices `count` American plans by least-squares Monte Carlo

@@ -33,8 +33,58 @@ thread_local uint32_t g_kernel_launches = 0;
 
 std::mutex g_init_mutex;
 bool g_initialised = false;
-hipStream_t g_stream = nullptr;
 int g_device = 0;
+
+// One HIP stream per entry-point call: a call takes a stream from the pool
+// (or creates one), makes it this thread's current stream, and returns it
+// when it ends, so concurrent calls from several host threads never share a
+// stream or a graph capture.
+std::mutex g_pool_mutex;
+std::vector<hipStream_t> g_free_streams;
+thread_local hipStream_t t_stream = nullptr;
+
+// A synchronous copy ordered on the call's stream (no legacy-stream
+// synchronisation with other threads' work).
+hipError_t copy_on_stream(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, t_stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(t_stream);
+}
+
+struct CallStream {
+    hipStream_t prev = nullptr, own = nullptr;
+    CallStream() {
+        {
+            std::lock_guard<std::mutex> lock(g_pool_mutex);
+            if (!g_free_streams.empty()) {
+                own = g_free_streams.back();
+                g_free_streams.pop_back();
+            }
+        }
+        if (!own && hipStreamCreateWithFlags(&own, hipStreamNonBlocking) != hipSuccess)
+            throw PlanError(PE_ERR_DEVICE, "hipStreamCreate failed");
+        prev = t_stream;
+        t_stream = own;
+    }
+    ~CallStream() {
+        t_stream = prev;
+        std::lock_guard<std::mutex> lock(g_pool_mutex);
+        g_free_streams.push_back(own);
+    }
+    CallStream(const CallStream&) = delete;
+    CallStream& operator=(const CallStream&) = delete;
+};
+
+// Locks the plans of one call (each plan's launch caches and device tables)
+// in address order, so calls on overlapping plan sets cannot deadlock.
+struct PlanLocks {
+    std::vector<std::unique_lock<std::mutex>> held;
+    explicit PlanLocks(std::vector<HostPlan*> ps) {
+        std::sort(ps.begin(), ps.end());
+        ps.erase(std::unique(ps.begin(), ps.end()), ps.end());
+        for (HostPlan* q : ps) held.emplace_back(q->mu);
+    }
+};
 
 pe_status fail(pe_status s, const std::string& what) {
     g_error = what;
@@ -101,7 +151,6 @@ void ensure_init() {
     check(hipGetDeviceCount(&count), "hipGetDeviceCount");
     if (count <= 0) throw PlanError(PE_ERR_DEVICE, "no HIP device");
     check(hipSetDevice(g_device), "hipSetDevice");
-    check(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking), "hipStreamCreate");
     check(scratch_setup(g_device), "workspace symbol address");
     g_initialised = true;
 }
@@ -147,6 +196,7 @@ void assemble(const HostPlan& p, const std::vector<BlockStat>& partials, uint32_
     out->time_steps = p.lv ? static_cast<uint32_t>(p.nodes.size() - 1) : static_cast<uint32_t>(p.obs_times.size());
     out->path_steps = out->evaluated_paths * out->time_steps;
     out->sobol_dimension = rqmc ? p.sobol_dims : 0;
+    out->diagnostics = p.diag;
 }
 
 dim3 grid_for(const pe_engine& e) {
@@ -166,9 +216,9 @@ float timed(F&& launch) {
     hipEvent_t a, b;
     check(hipEventCreate(&a), "hipEventCreate");
     check(hipEventCreate(&b), "hipEventCreate");
-    check(hipEventRecord(a, g_stream), "hipEventRecord");
+    check(hipEventRecord(a, t_stream), "hipEventRecord");
     launch();
-    check(hipEventRecord(b, g_stream), "hipEventRecord");
+    check(hipEventRecord(b, t_stream), "hipEventRecord");
     check(hipEventSynchronize(b), "hipEventSynchronize");
     float ms = 0.0f;
     check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
@@ -331,26 +381,32 @@ BsLaunch& prepare_bs(std::vector<HostPlan*>& plans, pe_backend backend) {
 
 // Enqueues the launch of every non-empty group in `mask` with the given
 // persistent grid sizes (0: one workgroup per item).
+BsBatchArgs bs_args(BsLaunch& L, int g, uint32_t grid) {
+    BsGroup& grp = L.groups[g];
+    BsBatchArgs a{};
+    a.trades = L.d_trades.as<BsTrade>();
+    a.obs = L.d_obs.as<BsObs>();
+    a.sobol = L.d_sobol.as<uint32_t>();
+    a.items = grp.d_items.as<BsItem>();
+    a.partials = static_cast<BlockStat*>(L.partials.ptr) + grp.partial_base;
+    a.error = static_cast<unsigned long long*>(L.err.ptr);
+    a.graph_ops = L.d_graph_ops.as<GraphOp>();
+    a.graph_segs = L.d_graph_segs.as<uint32_t>();
+    a.graph_consts = L.d_graph_consts.as<double>();
+    a.n_items = static_cast<uint32_t>(grp.items.size());
+    a.components = grp.components;
+    a.grid = grid;
+    return a;
+}
+
 uint32_t enqueue_bs_batch(BsLaunch& L, pe_backend backend, const uint32_t (&grid)[kBsGroups], uint32_t mask = 31u) {
     uint32_t launches = 0;
     for (int g = 0; g < kBsGroups; ++g) {
         BsGroup& grp = L.groups[g];
         if (grp.items.empty() || !((mask >> g) & 1u)) continue;
-        BsBatchArgs a{};
-        a.trades = L.d_trades.as<BsTrade>();
-        a.obs = L.d_obs.as<BsObs>();
-        a.sobol = L.d_sobol.as<uint32_t>();
-        a.items = grp.d_items.as<BsItem>();
-        a.partials = static_cast<BlockStat*>(L.partials.ptr) + grp.partial_base;
-        a.error = static_cast<unsigned long long*>(L.err.ptr);
-        a.graph_ops = L.d_graph_ops.as<GraphOp>();
-        a.graph_segs = L.d_graph_segs.as<uint32_t>();
-        a.graph_consts = L.d_graph_consts.as<double>();
-        a.n_items = static_cast<uint32_t>(grp.items.size());
-        a.components = grp.components;
-        a.grid = grid[g];
-        check(backend == PE_BACKEND_GPU_FP64 ? launch_bs_batch_f64(a, grp.risk, grp.cont, grp.graph_regs, g_stream)
-                                             : launch_bs_batch_f32(a, grp.risk, grp.cont, grp.graph_regs, g_stream),
+        const BsBatchArgs a = bs_args(L, g, grid[g]);
+        check(backend == PE_BACKEND_GPU_FP64 ? launch_bs_batch_f64(a, grp.risk, grp.cont, grp.graph_regs, t_stream)
+                                             : launch_bs_batch_f32(a, grp.risk, grp.cont, grp.graph_regs, t_stream),
               "bs kernel launch");
         ++launches;
     }
@@ -397,22 +453,30 @@ void evaluate_bs_batch(std::vector<HostPlan*>& plans, pe_backend backend, pe_res
     BsLaunch& L = prepare_bs(plans, backend);
     const unsigned long long none = ~0ull;
     if (!L.tuned) {
-        check(hipMemcpyAsync(L.err.ptr, &none, sizeof(none), hipMemcpyHostToDevice, g_stream), "error word");
+        check(hipMemcpyAsync(L.err.ptr, &none, sizeof(none), hipMemcpyHostToDevice, t_stream), "error word");
         tune_bs_batch(L, backend);
-        check(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal), "graph capture");
-        enqueue_bs_batch(L, backend, L.grid);
-        check(hipStreamEndCapture(g_stream, &L.graph), "graph capture");
+        // the tuned launches as kernel nodes of one graph, built explicitly
+        // (no stream capture, so other threads' calls never meet a capture)
+        check(hipGraphCreate(&L.graph, 0), "graph create");
+        for (int g = 0; g < kBsGroups; ++g) {
+            if (L.groups[g].items.empty()) continue;
+            const BsBatchArgs a = bs_args(L, g, L.grid[g]);
+            const BsGroup& grp = L.groups[g];
+            check(backend == PE_BACKEND_GPU_FP64 ? add_bs_batch_node_f64(L.graph, a, grp.risk, grp.cont, grp.graph_regs)
+                                                 : add_bs_batch_node_f32(L.graph, a, grp.risk, grp.cont, grp.graph_regs),
+                  "graph kernel node");
+        }
         check(hipGraphInstantiate(&L.graph_exec, L.graph, nullptr, nullptr, 0), "graph instantiate");
     }
-    check(hipMemcpyAsync(L.err.ptr, &none, sizeof(none), hipMemcpyHostToDevice, g_stream), "error word");
+    check(hipMemcpyAsync(L.err.ptr, &none, sizeof(none), hipMemcpyHostToDevice, t_stream), "error word");
     uint32_t launches = 0;
     for (int g = 0; g < kBsGroups; ++g) launches += L.groups[g].items.empty() ? 0u : 1u;
-    const float ms = timed([&] { check(hipGraphLaunch(L.graph_exec, g_stream), "graph launch"); });
-    check(hipMemcpy(L.host_partials.data(), L.partials.ptr, L.partial_count * sizeof(BlockStat),
+    const float ms = timed([&] { check(hipGraphLaunch(L.graph_exec, t_stream), "graph launch"); });
+    check(copy_on_stream(L.host_partials.data(), L.partials.ptr, L.partial_count * sizeof(BlockStat),
                     hipMemcpyDeviceToHost),
           "partials copy");
     unsigned long long err = none;
-    check(hipMemcpy(&err, L.err.ptr, sizeof(err), hipMemcpyDeviceToHost), "error copy");
+    check(copy_on_stream(&err, L.err.ptr, sizeof(err), hipMemcpyDeviceToHost), "error copy");
     if (err != none && (err >> 63)) {
         const uint64_t trade = (err >> 40) & ((1ull << 23) - 1), unit = err & ((1ull << 40) - 1);
         throw PlanError(PE_ERR_NONFINITE_STATE, "trade " + std::to_string(trade) + ", sampling unit " +
@@ -504,7 +568,7 @@ void evaluate_bs_noalloc(HostPlan& p, pe_backend backend, pe_result* out) {
             it.partial = sc * chunks + c;
         }
     auto put = [&](size_t at, const void* src, size_t bytes) {
-        if (bytes) check(hipMemcpyAsync(base + at, src, bytes, hipMemcpyHostToDevice, g_stream), "workspace upload");
+        if (bytes) check(hipMemcpyAsync(base + at, src, bytes, hipMemcpyHostToDevice, t_stream), "workspace upload");
     };
     const unsigned long long none = ~0ull;
     put(o_err, &none, sizeof(none));
@@ -531,16 +595,16 @@ void evaluate_bs_noalloc(HostPlan& p, pe_backend backend, pe_result* out) {
     a.components = comps;
     a.grid = 0;
     const uint32_t graph_regs = p.graph ? gt.regs : 0u;
-    check(backend == PE_BACKEND_GPU_FP64 ? launch_bs_batch_f64(a, risk, cont, graph_regs, g_stream)
-                                         : launch_bs_batch_f32(a, risk, cont, graph_regs, g_stream),
+    check(backend == PE_BACKEND_GPU_FP64 ? launch_bs_batch_f64(a, risk, cont, graph_regs, t_stream)
+                                         : launch_bs_batch_f32(a, risk, cont, graph_regs, t_stream),
           "bs kernel launch");
     std::vector<BlockStat> partials(static_cast<size_t>(n_items) * comps);
     unsigned long long err = none;
     check(hipMemcpyAsync(partials.data(), base + o_partials, partials.size() * sizeof(BlockStat),
-                         hipMemcpyDeviceToHost, g_stream),
+                         hipMemcpyDeviceToHost, t_stream),
           "partials copy");
-    check(hipMemcpyAsync(&err, base + o_err, sizeof(err), hipMemcpyDeviceToHost, g_stream), "error copy");
-    check(hipStreamSynchronize(g_stream), "stream synchronize");
+    check(hipMemcpyAsync(&err, base + o_err, sizeof(err), hipMemcpyDeviceToHost, t_stream), "error copy");
+    check(hipStreamSynchronize(t_stream), "stream synchronize");
     if (err != none) {
         const uint64_t unit = err & ((1ull << 40) - 1);
         out->failing_unit = unit;
@@ -588,19 +652,19 @@ void evaluate_rough(std::vector<HostPlan*>& plans, pe_backend backend, pe_result
         prepare_rough(plans, backend);
     RoughLaunch& L = *p0.rough_launch;
     const unsigned long long none = ~0ull;
-    check(hipMemcpyAsync(L.d_err.ptr, &none, sizeof(none), hipMemcpyHostToDevice, g_stream), "error word");
+    check(hipMemcpyAsync(L.d_err.ptr, &none, sizeof(none), hipMemcpyHostToDevice, t_stream), "error word");
     const float ms = timed([&] {
         check(backend == PE_BACKEND_GPU_FP64
                   ? launch_rough_f64(L.a, L.pa, L.grid, static_cast<uint32_t>(L.items),
-                                     static_cast<unsigned long long*>(L.d_err.ptr), g_stream)
+                                     static_cast<unsigned long long*>(L.d_err.ptr), t_stream)
                   : launch_rough_f32(L.a, L.pa, L.grid, static_cast<uint32_t>(L.items),
-                                     static_cast<unsigned long long*>(L.d_err.ptr), g_stream),
+                                     static_cast<unsigned long long*>(L.d_err.ptr), t_stream),
               "rough kernel launch");
     });
-    check(hipMemcpy(L.host.data(), L.d_partials.ptr, L.host.size() * sizeof(BlockStat), hipMemcpyDeviceToHost),
+    check(copy_on_stream(L.host.data(), L.d_partials.ptr, L.host.size() * sizeof(BlockStat), hipMemcpyDeviceToHost),
           "partials");
     unsigned long long err = none;
-    check(hipMemcpy(&err, L.d_err.ptr, sizeof(err), hipMemcpyDeviceToHost), "error word");
+    check(copy_on_stream(&err, L.d_err.ptr, sizeof(err), hipMemcpyDeviceToHost), "error word");
     if (err != none)
         throw PlanError(PE_ERR_NONFINITE_STATE, "sampling unit " + std::to_string(err) + ": non-finite lifted Heston path");
     const size_t per = static_cast<size_t>(L.scrambles) * L.chunks;
@@ -990,21 +1054,21 @@ void evaluate_lv(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* o
     LvLaunch& L = prepare_lv(plans, backend);
     const int no_error = 0x7fffffff;
     *L.host_err = no_error;
-    check(hipMemcpyAsync(L.err.ptr, L.host_err, sizeof(int), hipMemcpyHostToDevice, g_stream), "error word init");
+    check(hipMemcpyAsync(L.err.ptr, L.host_err, sizeof(int), hipMemcpyHostToDevice, t_stream), "error word init");
     const float ms = timed([&] {
         if (L.risk)
-            check(backend == PE_BACKEND_GPU_FP64 ? launch_lv_risk_f64(L.args, L.grid, L.lds, L.err.as<int>(), g_stream)
-                                                 : launch_lv_risk_f32(L.args, L.grid, L.lds, L.err.as<int>(), g_stream),
+            check(backend == PE_BACKEND_GPU_FP64 ? launch_lv_risk_f64(L.args, L.grid, L.lds, L.err.as<int>(), t_stream)
+                                                 : launch_lv_risk_f32(L.args, L.grid, L.lds, L.err.as<int>(), t_stream),
                   "lv risk kernel launch");
         else
-            check(backend == PE_BACKEND_GPU_FP64 ? launch_lv_f64(L.args, L.grid, L.lds, L.err.as<int>(), g_stream)
-                                                 : launch_lv_f32(L.args, L.grid, L.lds, L.err.as<int>(), g_stream),
+            check(backend == PE_BACKEND_GPU_FP64 ? launch_lv_f64(L.args, L.grid, L.lds, L.err.as<int>(), t_stream)
+                                                 : launch_lv_f32(L.args, L.grid, L.lds, L.err.as<int>(), t_stream),
                   "lv kernel launch");
     });
     check(hipMemcpyAsync(L.host_partials, L.partials.ptr, L.partial_count * sizeof(BlockStat), hipMemcpyDeviceToHost,
-                         g_stream), "partials copy");
-    check(hipMemcpyAsync(L.host_err, L.err.ptr, sizeof(int), hipMemcpyDeviceToHost, g_stream), "error word copy");
-    check(hipStreamSynchronize(g_stream), "stream synchronize");
+                         t_stream), "partials copy");
+    check(hipMemcpyAsync(L.host_err, L.err.ptr, sizeof(int), hipMemcpyDeviceToHost, t_stream), "error word copy");
+    check(hipStreamSynchronize(t_stream), "stream synchronize");
     const int failing = *L.host_err;
     if (failing != no_error) {
         for (size_t k = 0; k < plans.size(); ++k) outs[k].failing_unit = static_cast<uint64_t>(failing);
@@ -1055,7 +1119,8 @@ struct LsmLaunch {
     std::vector<LsmItem> train_items, value_items;
     std::vector<uint32_t> first_partial, chunks, scrambles;
     DeviceBuffer d_trades, d_dates, d_sobol, d_train_items, d_value_items, store, cash, decisions, counts, in_sample,
-        partials, risk_partials, err;
+        partials, risk_partials, err, stops;
+    uint64_t n_trajectories = 0;             // valuation trajectories of all trades
     size_t n_decisions = 0, n_counts = 0, n_partials = 0;
     bool risk = false;                       // some plan asks for fixed-policy Greeks
     std::vector<BlockStat> host_partials, host_risk_partials;
@@ -1186,6 +1251,8 @@ LsmLaunch& prepare_lsm(std::vector<HostPlan*>& plans, pe_backend backend) {
         t.count_offset = static_cast<uint32_t>(L->n_counts);
         L->n_decisions += n - 1;
         L->n_counts += n;
+        t.stop_offset = L->n_trajectories;
+        L->n_trajectories += t.value.units * t.value.scrambles * (t.value.antithetic ? 2u : 1u);
         for (LsmEngine* e : {&t.train, &t.value}) {
             if (!e->rqmc) continue;
             e->sobol_offset = static_cast<uint32_t>(sobol.size());
@@ -1278,7 +1345,7 @@ void evaluate_lsm_dual(HostPlan* plan, uint32_t n2, uint32_t n3, uint64_t seed, 
     std::vector<HostPlan*> plans{plan};
     LsmLaunch& L = prepare_lsm(plans, PE_BACKEND_GPU_FP64);
     const unsigned long long none = ~0ull;
-    check(hipMemcpyAsync(L.err.ptr, &none, sizeof(none), hipMemcpyHostToDevice, g_stream), "error word");
+    check(hipMemcpyAsync(L.err.ptr, &none, sizeof(none), hipMemcpyHostToDevice, t_stream), "error word");
     LsmArgs a{};
     a.trades = L.d_trades.as<LsmTrade>();
     a.dates = L.d_dates.as<LsmDate>();
@@ -1308,15 +1375,15 @@ void evaluate_lsm_dual(HostPlan* plan, uint32_t n2, uint32_t n3, uint64_t seed, 
     da.n3 = n3;
     da.lanes = static_cast<uint32_t>(lanes);
     const float ms = timed([&] {
-        check(launch_lsm_f64(train, train.n_items, train, 0, L.basis, false, g_stream, nullptr, nullptr),
+        check(launch_lsm_f64(train, train.n_items, train, 0, L.basis, false, t_stream, nullptr, nullptr),
               "LSM training launch");
-        check(launch_lsm_dual_f64(da, pairs, blocks, L.basis, g_stream), "Andersen-Broadie launch");
+        check(launch_lsm_dual_f64(da, pairs, blocks, L.basis, t_stream), "Andersen-Broadie launch");
     });
     unsigned long long err = none;
-    check(hipMemcpy(&err, L.err.ptr, sizeof(err), hipMemcpyDeviceToHost), "error copy");
+    check(copy_on_stream(&err, L.err.ptr, sizeof(err), hipMemcpyDeviceToHost), "error copy");
     if (err != none) throw PlanError(PE_ERR_NONFINITE_STATE, "non-finite early-exercise training path");
     std::vector<BlockStat> host(static_cast<size_t>(blocks) * 3);
-    check(hipMemcpy(host.data(), partials.ptr, host.size() * sizeof(BlockStat), hipMemcpyDeviceToHost),
+    check(copy_on_stream(host.data(), partials.ptr, host.size() * sizeof(BlockStat), hipMemcpyDeviceToHost),
           "partials copy");
     pe_result r{};
     assemble(p, host, blocks, 1, 3, 0, 3, &r);
@@ -1337,8 +1404,8 @@ void evaluate_lsm_dual(HostPlan* plan, uint32_t n2, uint32_t n3, uint64_t seed, 
 void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* outs, pe_lsm_report* reports) {
     LsmLaunch& L = prepare_lsm(plans, backend);
     const unsigned long long none = ~0ull;
-    check(hipMemcpyAsync(L.err.ptr, &none, sizeof(none), hipMemcpyHostToDevice, g_stream), "error word");
-    check(hipMemsetAsync(L.counts.ptr, 0, L.n_counts * sizeof(unsigned long long), g_stream), "counts");
+    check(hipMemcpyAsync(L.err.ptr, &none, sizeof(none), hipMemcpyHostToDevice, t_stream), "error word");
+    check(hipMemsetAsync(L.counts.ptr, 0, L.n_counts * sizeof(unsigned long long), t_stream), "counts");
     LsmArgs a{};
     a.trades = L.d_trades.as<LsmTrade>();
     a.dates = L.d_dates.as<LsmDate>();
@@ -1352,7 +1419,15 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
     a.risk_partials = L.risk ? L.risk_partials.ptr : nullptr;
     a.error = L.err.as<unsigned long long>();
     a.n_trades = static_cast<uint32_t>(plans.size());
+    bool want_stops = false;
+    for (size_t ti = 0; reports && ti < plans.size(); ++ti)
+        want_stops = want_stops || (reports[ti].stopping_indices && reports[ti].stopping_capacity);
+    if (want_stops) {
+        L.stops.reserve(std::max<uint64_t>(1, L.n_trajectories) * sizeof(uint32_t));
+        a.stops = L.stops.as<uint32_t>();
+    }
     LsmArgs train = a, value = a;
+    train.stops = nullptr;
     train.items = L.d_train_items.as<LsmItem>();
     train.n_items = static_cast<uint32_t>(L.train_items.size());
     value.items = L.d_value_items.as<LsmItem>();
@@ -1361,49 +1436,49 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
         train.cuts = value.cuts = L.cuts.as<LsmCuts>();
         L.mlp_args.lsm = train;
         check(hipMemcpyAsync(L.weights.ptr, L.mlp_init.data(), kMlpParameterCount * sizeof(float),
-                             hipMemcpyHostToDevice, g_stream),
+                             hipMemcpyHostToDevice, t_stream),
               "MLP weights");
-        check(hipMemsetAsync(L.adam_m.ptr, 0, kMlpParameterCount * sizeof(float), g_stream), "Adam state");
-        check(hipMemsetAsync(L.adam_v.ptr, 0, kMlpParameterCount * sizeof(float), g_stream), "Adam state");
-        check(hipMemsetAsync(L.mlp_err.ptr, 0, sizeof(unsigned int), g_stream), "MLP error word");
-        check(hipMemsetAsync(L.cuts.ptr, 0, std::max<size_t>(1, L.n_decisions) * sizeof(LsmCuts), g_stream), "cuts");
+        check(hipMemsetAsync(L.adam_m.ptr, 0, kMlpParameterCount * sizeof(float), t_stream), "Adam state");
+        check(hipMemsetAsync(L.adam_v.ptr, 0, kMlpParameterCount * sizeof(float), t_stream), "Adam state");
+        check(hipMemsetAsync(L.mlp_err.ptr, 0, sizeof(unsigned int), t_stream), "MLP error word");
+        check(hipMemsetAsync(L.cuts.ptr, 0, std::max<size_t>(1, L.n_decisions) * sizeof(LsmCuts), t_stream), "cuts");
     }
-    check(hipEventRecord(L.ev[0], g_stream), "hipEventRecord");
+    check(hipEventRecord(L.ev[0], t_stream), "hipEventRecord");
     if (L.mlp)
         check(backend == PE_BACKEND_GPU_FP64
-                  ? launch_lsm_mlp_f64(train, train.n_items, L.mlp_args, L.mlp_wgs, value, value.n_items, L.risk, g_stream,
+                  ? launch_lsm_mlp_f64(train, train.n_items, L.mlp_args, L.mlp_wgs, value, value.n_items, L.risk, t_stream,
                                        L.ev[3], L.ev[1])
-                  : launch_lsm_mlp_f32(train, train.n_items, L.mlp_args, L.mlp_wgs, value, value.n_items, L.risk, g_stream,
+                  : launch_lsm_mlp_f32(train, train.n_items, L.mlp_args, L.mlp_wgs, value, value.n_items, L.risk, t_stream,
                                        L.ev[3], L.ev[1]),
               "MLP LSM launch");
     else
         check(backend == PE_BACKEND_GPU_FP64
-                  ? launch_lsm_f64(train, train.n_items, value, value.n_items, L.basis, L.risk, g_stream, L.ev[3], L.ev[1])
-                  : launch_lsm_f32(train, train.n_items, value, value.n_items, L.basis, L.risk, g_stream, L.ev[3], L.ev[1]),
+                  ? launch_lsm_f64(train, train.n_items, value, value.n_items, L.basis, L.risk, t_stream, L.ev[3], L.ev[1])
+                  : launch_lsm_f32(train, train.n_items, value, value.n_items, L.basis, L.risk, t_stream, L.ev[3], L.ev[1]),
               "LSM kernel launch");
-    check(hipEventRecord(L.ev[2], g_stream), "hipEventRecord");
+    check(hipEventRecord(L.ev[2], t_stream), "hipEventRecord");
     check(hipEventSynchronize(L.ev[2]), "hipEventSynchronize");
     float train_ms = 0.0f, value_ms = 0.0f, path_ms = 0.0f;
     check(hipEventElapsedTime(&train_ms, L.ev[0], L.ev[1]), "hipEventElapsedTime");
     check(hipEventElapsedTime(&path_ms, L.ev[0], L.ev[3]), "hipEventElapsedTime");
     check(hipEventElapsedTime(&value_ms, L.ev[1], L.ev[2]), "hipEventElapsedTime");
-    check(hipMemcpy(L.host_partials.data(), L.partials.ptr, L.n_partials * sizeof(BlockStat), hipMemcpyDeviceToHost),
+    check(copy_on_stream(L.host_partials.data(), L.partials.ptr, L.n_partials * sizeof(BlockStat), hipMemcpyDeviceToHost),
           "partials copy");
     if (L.risk)
-        check(hipMemcpy(L.host_risk_partials.data(), L.risk_partials.ptr,
+        check(copy_on_stream(L.host_risk_partials.data(), L.risk_partials.ptr,
                         L.host_risk_partials.size() * sizeof(BlockStat), hipMemcpyDeviceToHost),
               "risk partials copy");
     if (L.n_decisions)
-        check(hipMemcpy(L.host_decisions.data(), L.decisions.ptr, L.n_decisions * sizeof(LsmDecision),
+        check(copy_on_stream(L.host_decisions.data(), L.decisions.ptr, L.n_decisions * sizeof(LsmDecision),
                         hipMemcpyDeviceToHost),
               "decisions copy");
-    check(hipMemcpy(L.host_counts.data(), L.counts.ptr, L.n_counts * sizeof(unsigned long long),
+    check(copy_on_stream(L.host_counts.data(), L.counts.ptr, L.n_counts * sizeof(unsigned long long),
                     hipMemcpyDeviceToHost),
           "counts copy");
-    check(hipMemcpy(L.host_in_sample.data(), L.in_sample.ptr, plans.size() * sizeof(double), hipMemcpyDeviceToHost),
+    check(copy_on_stream(L.host_in_sample.data(), L.in_sample.ptr, plans.size() * sizeof(double), hipMemcpyDeviceToHost),
           "in-sample copy");
     unsigned long long err = none;
-    check(hipMemcpy(&err, L.err.ptr, sizeof(err), hipMemcpyDeviceToHost), "error copy");
+    check(copy_on_stream(&err, L.err.ptr, sizeof(err), hipMemcpyDeviceToHost), "error copy");
     if (err != none) {
         const uint64_t trade = err >> 40, unit = err & ((1ull << 40) - 1);
         throw PlanError(PE_ERR_NONFINITE_STATE, "trade " + std::to_string(trade) + ", sampling unit " +
@@ -1411,10 +1486,10 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
     }
     if (L.mlp) {
         unsigned int merr = 0;
-        check(hipMemcpy(&merr, L.mlp_err.ptr, sizeof(merr), hipMemcpyDeviceToHost), "MLP error copy");
+        check(copy_on_stream(&merr, L.mlp_err.ptr, sizeof(merr), hipMemcpyDeviceToHost), "MLP error copy");
         if (merr) throw PlanError(PE_ERR_SIZE_LIMIT, "an MLP exercise region needs more moneyness cuts than the grid allows");
         if (L.n_decisions)
-            check(hipMemcpy(L.host_cuts.data(), L.cuts.ptr, L.n_decisions * sizeof(LsmCuts), hipMemcpyDeviceToHost),
+            check(copy_on_stream(L.host_cuts.data(), L.cuts.ptr, L.n_decisions * sizeof(LsmCuts), hipMemcpyDeviceToHost),
                   "cuts copy");
     }
     for (size_t ti = 0; ti < plans.size(); ++ti) {
@@ -1449,7 +1524,39 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
         r.valuation_ms = value_ms;
         r.path_ms = path_ms;
         r.exercise_dates = t.n_dates;
+        // EarlyExerciseDiagnostics (risk/early_exercise.rs:109-113, 190-210)
+        const uint64_t vtraj = t.value.units * t.value.scrambles * (t.value.antithetic ? 2u : 1u);
+        // RQMC engines draw from the scramble domain (the reported RandomDomain)
+        r.training_random_domain = t.train.rqmc ? PE_DOMAIN_RQMC_SCRAMBLE : PE_DOMAIN_LSM_TRAIN;
+        r.valuation_random_domain = t.value.rqmc ? PE_DOMAIN_RQMC_SCRAMBLE : PE_DOMAIN_VALUATION;
+        // sampling units are the independent units: pseudo-MC units, or RQMC scrambles
+        r.training_sampling_units = t.train.rqmc ? t.train.scrambles : t.train.units;
+        r.training_trajectories = t.train_paths;
+        r.valuation_sampling_units = t.value.rqmc ? t.value.scrambles : t.value.units;
+        r.valuation_trajectories = vtraj;
+        r.training_has_checksums = p.train_diag.has_checksums;
+        std::memcpy(r.training_direction_checksum, p.train_diag.direction_checksum, 32);
+        std::memcpy(r.training_scramble_checksum, p.train_diag.scramble_checksum, 32);
+        r.valuation_has_checksums = p.diag.has_checksums;
+        std::memcpy(r.valuation_direction_checksum, p.diag.direction_checksum, 32);
+        std::memcpy(r.valuation_scramble_checksum, p.diag.scramble_checksum, 32);
+        r.policy_feature_count = 1;
+        r.policy_max_degree = p.req.lsm.max_degree;
+        r.itm_abs_tolerance = p.req.lsm.itm_abs_tolerance;
+        r.abs_rank_tolerance = p.req.lsm.abs_rank_tolerance;
+        r.rel_rank_tolerance = p.req.lsm.rel_rank_tolerance;
+        r.max_matrix_elements = p.req.lsm.max_matrix_elements;
+        if (r.stopping_indices && r.stopping_capacity) {
+            const uint64_t k = std::min<uint64_t>(r.stopping_capacity, vtraj);
+            check(copy_on_stream(r.stopping_indices, L.stops.as<uint32_t>() + t.stop_offset, k * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost),
+                  "stopping indices copy");
+        }
         for (uint32_t d = 0; d < t.n_dates && d < r.capacity; ++d) {
+            if (r.exercise_probabilities)
+                r.exercise_probabilities[d] =
+                    static_cast<double>(L.host_counts[t.count_offset + d]) / static_cast<double>(vtraj);
+            if (r.dividend_collisions) r.dividend_collisions[d] = p.lsm_collisions[d];
             if (r.exercise_counts) r.exercise_counts[d] = L.host_counts[t.count_offset + d];
             if (!r.decisions || d + 1 >= t.n_dates) continue;
             const LsmDecision& s = L.host_decisions[t.decision_offset + d];
@@ -1498,6 +1605,9 @@ pe_status guarded(F&& f) {
 }
 
 }  // namespace
+
+hipStream_t current_stream() { return t_stream; }
+
 }  // namespace pe
 
 using namespace pe;
@@ -1527,12 +1637,28 @@ pe_status pe_plan_compile(const pe_request* request, const pe_exec_policy* polic
 
 void pe_plan_free(pe_plan* plan) { delete plan; }
 
+pe_status pe_plan_diagnostics(const pe_plan* plan, pe_mc_diagnostics* out) {
+    if (!plan || !out) return fail(PE_ERR_INVALID_ARGUMENT, "null argument");
+    *out = plan->host.diag;
+    return PE_OK;
+}
+
+pe_status pe_blake3_256(const void* data, size_t bytes, uint8_t out[32]) {
+    if ((!data && bytes) || !out) return fail(PE_ERR_INVALID_ARGUMENT, "null argument");
+    Blake3 b;
+    b.update(data, bytes);
+    b.finalize(out);
+    return PE_OK;
+}
+
 pe_status pe_plan_evaluate(const pe_plan* plan, pe_backend backend, pe_result* out) {
     if (!plan || !out) return fail(PE_ERR_INVALID_ARGUMENT, "null argument");
     if (backend != PE_BACKEND_GPU_FP64 && backend != PE_BACKEND_GPU_MIXED) return fail(PE_ERR_INVALID_ARGUMENT, "backend");
     return guarded([&] {
         ensure_init();
         HostPlan& p = const_cast<pe_plan*>(plan)->host;
+        CallStream cs;
+        PlanLocks locks({&p});
         if (p.rough) {
             std::vector<HostPlan*> plans{&p};
             evaluate_rough(plans, backend, out);
@@ -1554,7 +1680,9 @@ pe_status pe_plan_evaluate_noalloc(const pe_plan* plan, pe_backend backend, pe_r
     if (backend != PE_BACKEND_GPU_FP64 && backend != PE_BACKEND_GPU_MIXED) return fail(PE_ERR_INVALID_ARGUMENT, "backend");
     return guarded([&] {
         ensure_init();
+        CallStream cs;
         std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+        PlanLocks locks({&const_cast<pe_plan*>(plan)->host});
         evaluate_bs_noalloc(const_cast<pe_plan*>(plan)->host, backend, out);
     });
 }
@@ -1569,6 +1697,8 @@ pe_status pe_plans_evaluate_shared(const pe_plan* const* plans, uint32_t count, 
             if (!plans[i]) throw PlanError(PE_ERR_INVALID_ARGUMENT, "null plan");
             hp.push_back(&const_cast<pe_plan*>(plans[i])->host);
         }
+        CallStream cs;
+        PlanLocks locks(hp);
         if (hp[0]->rough) evaluate_rough(hp, backend, out);
         else evaluate_lv(hp, backend, out);
     });
@@ -1605,6 +1735,8 @@ pe_status pe_batch_evaluate(const pe_plan* const* plans, uint32_t count, pe_back
             if (!plans[i]) throw PlanError(PE_ERR_INVALID_ARGUMENT, "null plan");
             hp.push_back(&const_cast<pe_plan*>(plans[i])->host);
         }
+        CallStream cs;
+        PlanLocks locks(hp);
         // American plans by exercise method, the rest through the BS batch
         std::vector<HostPlan*> groups[3], rest;
         std::vector<size_t> where[3], rest_at;
@@ -1643,6 +1775,8 @@ pe_status pe_lsm_evaluate(const pe_plan* const* plans, uint32_t count, pe_backen
             if (!plans[i]) throw PlanError(PE_ERR_INVALID_ARGUMENT, "null plan");
             hp.push_back(&const_cast<pe_plan*>(plans[i])->host);
         }
+        CallStream cs;
+        PlanLocks locks(hp);
         evaluate_lsm(hp, backend, results, reports);
     });
 }
@@ -1654,6 +1788,8 @@ pe_status pe_lsm_dual_evaluate(const pe_plan* plan, uint32_t continuation_inner_
     if (backend != PE_BACKEND_GPU_FP64) return fail(PE_ERR_UNSUPPORTED, "Andersen-Broadie runs in the FP64 backend");
     return guarded([&] {
         ensure_init();
+        CallStream cs;
+        PlanLocks locks({&const_cast<pe_plan*>(plan)->host});
         evaluate_lsm_dual(&const_cast<pe_plan*>(plan)->host, continuation_inner_paths, exercise_inner_paths,
                           inner_seed, out);
     });
@@ -1664,6 +1800,7 @@ pe_status pe_mlp_gradients(const float* params, const float* x, const float* y, 
     if (!params || !x || !y || !grad_out || rows == 0) return fail(PE_ERR_INVALID_ARGUMENT, "null argument");
     return guarded([&] {
         ensure_init();
+        CallStream cs;
         const uint32_t rows_wg = 256;
         const uint32_t wgs = (rows + rows_wg - 1) / rows_wg;
         const size_t np = kMlpParameterCount + 1;
@@ -1673,12 +1810,12 @@ pe_status pe_mlp_gradients(const float* params, const float* x, const float* y, 
         dy.upload(std::vector<float>(y, y + rows));
         dpart.reserve(static_cast<size_t>(wgs) * np * sizeof(float));
         check(launch_mlp_gradient(dp.as<float>(), dx.as<float>(), dy.as<float>(), rows, rows_wg, wgs,
-                                  dpart.as<float>(), g_stream),
+                                  dpart.as<float>(), t_stream),
               "MLP gradient launch");
         std::vector<float> part(static_cast<size_t>(wgs) * np);
-        check(hipMemcpyAsync(part.data(), dpart.ptr, part.size() * sizeof(float), hipMemcpyDeviceToHost, g_stream),
+        check(hipMemcpyAsync(part.data(), dpart.ptr, part.size() * sizeof(float), hipMemcpyDeviceToHost, t_stream),
               "partials copy");
-        check(hipStreamSynchronize(g_stream), "hipStreamSynchronize");
+        check(hipStreamSynchronize(t_stream), "hipStreamSynchronize");
         for (size_t k = 0; k < np; ++k) {
             double sum = 0.0;
             for (uint32_t w = 0; w < wgs; ++w) sum += part[static_cast<size_t>(w) * np + k];
@@ -1692,16 +1829,17 @@ pe_status pe_mlp_forward(const float* params, const float* x, uint32_t rows, flo
     if (!params || !x || !out || rows == 0) return fail(PE_ERR_INVALID_ARGUMENT, "null argument");
     return guarded([&] {
         ensure_init();
+        CallStream cs;
         DeviceBuffer dp, dx, dout;
         dp.upload(std::vector<float>(params, params + kMlpParameterCount));
         dx.upload(std::vector<float>(x, x + 2 * static_cast<size_t>(rows)));
         dout.reserve(static_cast<size_t>(rows) * sizeof(float));
-        check(launch_mlp_forward(dp.as<float>(), dx.as<float>(), rows, dout.as<float>(), g_stream),
+        check(launch_mlp_forward(dp.as<float>(), dx.as<float>(), rows, dout.as<float>(), t_stream),
               "MLP forward launch");
         check(hipMemcpyAsync(out, dout.ptr, static_cast<size_t>(rows) * sizeof(float), hipMemcpyDeviceToHost,
-                             g_stream),
+                             t_stream),
               "output copy");
-        check(hipStreamSynchronize(g_stream), "hipStreamSynchronize");
+        check(hipStreamSynchronize(t_stream), "hipStreamSynchronize");
     });
 }
 

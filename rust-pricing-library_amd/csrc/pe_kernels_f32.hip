@@ -15,6 +15,10 @@
 
 namespace pe {
 
+hipError_t add_bs_batch_node_f32(hipGraph_t graph, const BsBatchArgs& a, bool risk, bool cont, uint32_t graph_regs) {
+    return add_bs_batch_node<float>(graph, a, risk, cont, graph_regs);
+}
+
 hipError_t launch_bs_batch_f32(const BsBatchArgs& a, bool risk, bool cont, uint32_t graph_regs, hipStream_t s) {
     return launch_bs_batch<float>(a, risk, cont, graph_regs, s);
 }

@@ -12,6 +12,10 @@
 
 namespace pe {
 
+hipError_t add_bs_batch_node_f64(hipGraph_t graph, const BsBatchArgs& a, bool risk, bool cont, uint32_t graph_regs) {
+    return add_bs_batch_node<double>(graph, a, risk, cont, graph_regs);
+}
+
 hipError_t launch_bs_batch_f64(const BsBatchArgs& a, bool risk, bool cont, uint32_t graph_regs, hipStream_t s) {
     return launch_bs_batch<double>(a, risk, cont, graph_regs, s);
 }

@@ -91,6 +91,14 @@ void HostCurve::init(const pe_curve& c) {
     times[0] = 0.0;
 }
 
+// CurveRegion of a query time (LogLinearDiscountCurve::evaluate, curve.rs:165-190).
+pe_curve_region HostCurve::region(double t) const {
+    if (t == 0.0) t = 0.0;
+    const auto it = std::lower_bound(times.begin(), times.end(), t);
+    if (it != times.end() && *it == t) return PE_CURVE_PILLAR;
+    return it != times.end() ? PE_CURVE_INTERPOLATED : PE_CURVE_RIGHT_EXTRAPOLATED;
+}
+
 double HostCurve::log_discount(double t) const {
     if (!std::isfinite(t) || t < 0.0) invalid("curve query time");
     if (t == 0.0) t = 0.0;
@@ -274,26 +282,63 @@ void compile_bridge_program(const std::vector<double>& t, std::vector<BridgeOp>&
 // ------------------------------------------------------------ Sobol table --
 // [scramble][dim][33]: LMS-scrambled direction words (MSB-first unit lower
 // triangular rows from Philox domain RqmcScramble) and the digital shift.
+extern "C" const unsigned char pe_jk_directions_begin[];
+extern "C" const unsigned char pe_jk_directions_end[];
+
+// compile_scramble (qmc.rs:107-133): digital shift and MSB-first unit lower
+// triangular rows of one (scramble, dimension) from Philox domain RqmcScramble.
+static uint32_t scramble_rows(uint64_t seed, uint32_t s, uint32_t d, uint32_t (&rows)[32]) {
+    auto word = [seed](uint64_t scramble, uint32_t dim) {
+        const U4 b = philox_block(seed, scramble, dim >> 2, 2u);
+        return lane_of(b, dim & 3u);
+    };
+    const uint32_t block = d * 32u;
+    const uint32_t shift = word(s, block);
+    rows[0] = 0x80000000u;
+    for (uint32_t r = 1; r < 32; ++r) {
+        const uint32_t raw = word(s, block + r);
+        rows[r] = (raw & (0xffffffffu << (32 - r))) | (1u << (31 - r));
+    }
+    return shift;
+}
+
+// RqmcPlan checksums (qmc.rs:236-259): BLAKE3 of the direction asset, and of
+// every scramble's shift and rows as big-endian words, scramble-major.
+void rqmc_checksums(uint64_t seed, uint32_t scrambles, uint32_t dims, uint8_t (&direction)[32],
+                    uint8_t (&scramble)[32]) {
+    static const std::vector<uint8_t> asset_hash = [] {
+        uint8_t h[32];
+        Blake3 b;
+        b.update(pe_jk_directions_begin, static_cast<size_t>(pe_jk_directions_end - pe_jk_directions_begin));
+        b.finalize(h);
+        return std::vector<uint8_t>(h, h + 32);
+    }();
+    std::copy(asset_hash.begin(), asset_hash.end(), direction);
+    Blake3 b;
+    auto be = [&](uint32_t v) {
+        const uint8_t bytes[4] = {static_cast<uint8_t>(v >> 24), static_cast<uint8_t>(v >> 16),
+                                  static_cast<uint8_t>(v >> 8), static_cast<uint8_t>(v)};
+        b.update(bytes, 4);
+    };
+    for (uint32_t s = 0; s < scrambles; ++s)
+        for (uint32_t d = 0; d < dims; ++d) {
+            uint32_t rows[32];
+            be(scramble_rows(seed, s, d, rows));
+            for (uint32_t r : rows) be(r);
+        }
+    b.finalize(scramble);
+}
+
 std::vector<uint32_t> build_sobol_table(uint64_t seed, uint32_t scrambles, uint32_t dims) {
     uint32_t available = 0;
     const uint32_t* words = direction_words(&available);
     if (dims == 0) invalid("zero Sobol dimension");
     if (dims > available) throw PlanError(PE_ERR_SIZE_LIMIT, "Sobol dimension exceeds the Joe--Kuo limit");
     std::vector<uint32_t> table(static_cast<size_t>(scrambles) * dims * 33u);
-    auto word = [seed](uint64_t scramble, uint32_t dim) {
-        const U4 b = philox_block(seed, scramble, dim >> 2, 2u);
-        return lane_of(b, dim & 3u);
-    };
     for (uint32_t s = 0; s < scrambles; ++s) {
         for (uint32_t d = 0; d < dims; ++d) {
             uint32_t rows[32];
-            const uint32_t block = d * 32u;
-            const uint32_t shift = word(s, block);
-            rows[0] = 0x80000000u;
-            for (uint32_t r = 1; r < 32; ++r) {
-                const uint32_t raw = word(s, block + r);
-                rows[r] = (raw & (0xffffffffu << (32 - r))) | (1u << (31 - r));
-            }
+            const uint32_t shift = scramble_rows(seed, s, d, rows);
             uint32_t* out = &table[(static_cast<size_t>(s) * dims + d) * 33u];
             const uint32_t* dir = words + static_cast<size_t>(d) * 32u;
             for (int k = 0; k < 32; ++k) {
@@ -522,6 +567,9 @@ void compile_rough_plan(HostPlan& p, const pe_request& carrier, const pe_lifted_
     for (size_t i = 0; i + 1 < p.rough_events.size(); ++i)
         estimated += std::ceil((p.rough_events[i + 1] - p.rough_events[i]) / maximum_step);
     if (!std::isfinite(estimated) || estimated > 65536.0) throw PlanError(PE_ERR_SIZE_LIMIT, "rough time grid resource limit");
+    // RQMC dimensions of the plan's own grid: two normals per step
+    const size_t steps = compile_time_grid(p.rough_events, maximum_step).size() - 1;
+    compile_diagnostics(p, policy, static_cast<uint32_t>(2 * steps));
 }
 
 // ------------------------------------------------------------------ plan --
@@ -581,6 +629,21 @@ void compile_lsm_trade(HostPlan& p) {
     };
     t.train = engine(te, 1u);   // SamplingDomain::LsmTrain
     t.value = engine(ve, 0u);   // SamplingDomain::Valuation
+    // EarlyExerciseDiagnostics inputs: an ex-dividend time equal to an
+    // exercise time (compile/simulation.rs:437-441), the training engine's
+    // RQMC checksums over the exercise-date dimensions
+    p.lsm_collisions.assign(n, 0);
+    for (size_t i = 0; i < n; ++i) {
+        const double et = static_cast<double>(p.obs_days[i]) / 365.0;
+        for (const auto& e : p.dividends.timeline)
+            if (e.ex_time == et) p.lsm_collisions[i] = 1;
+    }
+    p.train_diag = pe_mc_diagnostics{};
+    if (te.kind == PE_ENGINE_RQMC) {
+        p.train_diag.has_checksums = 1;
+        rqmc_checksums(te.seed, te.scrambles, static_cast<uint32_t>(n), p.train_diag.direction_checksum,
+                       p.train_diag.scramble_checksum);
+    }
     t.train_paths = train_paths;
     t.n_dates = static_cast<uint32_t>(n);
     t.call = p.req.product.side == PE_CALL ? 1u : 0u;
@@ -898,6 +961,40 @@ void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* p
     p.sobol_dims = p.lv ? static_cast<uint32_t>(p.nodes.size() - 1) : static_cast<uint32_t>(p.obs_times.size());
     if (!p.lv) compile_bs_trade(p, monitoring_end);
     if (p.american) compile_lsm_trade(p);
+    compile_diagnostics(p, policy, p.sobol_dims);
+}
+
+// MonteCarloDiagnostics of the plan (valuation.rs:94-118 fills the same
+// fields): the engine identity, RQMC checksums, the execution, AAD-tile and
+// checkpoint policies (AadTilePolicy::resolve / CheckpointPolicy::resolve,
+// crates/pricing/src/engine/aad/workspace.rs:66-140) and the curve regions
+// of the payment and expiry times (compile/simulation.rs:267-276).
+void compile_diagnostics(HostPlan& p, const pe_exec_policy* policy, uint32_t dims) {
+    const pe_request& r = p.req;
+    pe_mc_diagnostics& d = p.diag;
+    d = pe_mc_diagnostics{};
+    const pe_engine& e = r.engine;
+    d.master_seed = e.seed;
+    d.estimator = e.kind == PE_ENGINE_RQMC ? PE_ESTIMATOR_RQMC : PE_ESTIMATOR_PSEUDO_MONTE_CARLO;
+    if (e.kind == PE_ENGINE_RQMC) {
+        d.scramble_count = e.scrambles;
+        d.has_checksums = 1;
+        rqmc_checksums(e.seed, e.scrambles, dims, d.direction_checksum, d.scramble_checksum);
+    }
+    d.policy_version = 1;
+    d.worker_threads = policy && policy->worker_threads ? policy->worker_threads : 1u;
+    d.reduction_block_size = p.block_size;
+    const uint64_t tile = r.risk.aad_tile_capacity ? r.risk.aad_tile_capacity : std::min<uint64_t>(256, p.block_size);
+    if (tile > p.block_size) invalid("AAD tile capacity exceeds the reduction block size");
+    d.aad_tile_policy_version = 1;
+    d.aad_tile_capacity = static_cast<uint32_t>(tile);
+    d.checkpoint_policy_version = 1;
+    d.checkpoint_interval = r.risk.checkpoint_interval ? r.risk.checkpoint_interval : 64u;
+    d.antithetic = e.antithetic ? 1 : 0;
+    const double expiry_time = p.obs_times.empty() ? 0.0 : static_cast<double>(r.product.expiry_day) / 365.0;
+    d.discount_region = p.discount_curve.region(static_cast<double>(r.product.payment_day) / 365.0);
+    d.dividend_region = p.dividend_curve.region(expiry_time);
+    d.valuation_kind = r.risk.smoothing ? PE_VALUATION_SMOOTHED_SURROGATE : PE_VALUATION_EXACT_CONTRACTUAL;
 }
 
 }  // namespace pe

@@ -9,6 +9,7 @@
 #include <climits>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -22,6 +23,10 @@ struct PlanError : std::runtime_error {
     pe_status status;
     PlanError(pe_status s, const std::string& what) : std::runtime_error(what), status(s) {}
 };
+
+// The calling thread's stream of the current entry-point call (nullptr
+// outside one: the legacy stream).
+hipStream_t current_stream();
 
 // Device buffer owned by a plan.
 struct DeviceBuffer {
@@ -48,7 +53,10 @@ struct DeviceBuffer {
     template <typename T>
     void upload(const std::vector<T>& v) {
         reserve(v.size() * sizeof(T) + 16);
-        if (!v.empty() && hipMemcpy(ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+        if (v.empty()) return;
+        hipStream_t s = current_stream();
+        if (hipMemcpyAsync(ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
             throw PlanError(PE_ERR_DEVICE, "hipMemcpy to device failed");
     }
     template <typename T>
@@ -60,6 +68,7 @@ struct HostCurve {
     std::vector<double> times, log_dfs;
     void init(const pe_curve& c);
     double log_discount(double t) const;
+    pe_curve_region region(double t) const;
 };
 
 // Launch state of a local-vol evaluation over one set of plans, cached on the
@@ -147,6 +156,7 @@ void compile_graph(const pe_product& pr, const std::function<double(int32_t)>& d
 
 struct HostPlan {
     uint64_t id = 0;                 // unique per compiled plan, never reused
+    std::mutex mu;                   // held by an evaluation that uses this plan's caches
     std::unique_ptr<LvLaunch> lv_launch;
     std::unique_ptr<BsLaunch, BsLaunchDeleter> bs_launch;
     std::unique_ptr<RoughLaunch, RoughLaunchDeleter> rough_launch;
@@ -169,6 +179,7 @@ struct HostPlan {
     double gamma_bump = 0.0, spot_bump = 0.0, vol_bump = 0.0;
     bool risk = false, request_gamma = false, request_delta = false, request_vega = false;
     uint64_t block_size = 4096;
+    pe_mc_diagnostics diag{};           // MonteCarloDiagnostics reported by every evaluation
 
     // local vol
     bool lv = false;
@@ -209,6 +220,8 @@ struct HostPlan {
     bool american = false;
     LsmTrade lsm_trade{};
     std::vector<LsmDate> lsm_dates;
+    std::vector<int32_t> lsm_collisions;  // per exercise date: an ex-dividend time equals it
+    pe_mc_diagnostics train_diag{};       // training engine's RQMC checksums
     std::unique_ptr<LsmLaunch, LsmLaunchDeleter> lsm_launch;
 
     // device tables
@@ -219,6 +232,9 @@ struct HostPlan {
 };
 
 void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* policy);
+void compile_diagnostics(HostPlan& p, const pe_exec_policy* policy, uint32_t sobol_dims);
+void rqmc_checksums(uint64_t seed, uint32_t scrambles, uint32_t dims, uint8_t (&direction)[32],
+                    uint8_t (&scramble)[32]);
 void compile_rough_plan(HostPlan& p, const pe_request& carrier, const pe_lifted_heston& m, double maximum_step,
                         const pe_exec_policy* policy);
 void lifted_from_rough(double hurst, uint32_t factors, double ratio, std::vector<double>& weights,
@@ -232,6 +248,25 @@ const uint32_t* direction_words(uint32_t* dimensions);  // embedded Joe--Kuo ass
 
 constexpr int32_t kNoDay = INT32_MIN;
 
+// BLAKE3-256 hasher (pe_blake3.cpp), unkeyed, incremental.
+class Blake3 {
+public:
+    Blake3();
+    void reset();
+    void update(const void* data, size_t n);
+    void finalize(uint8_t out[32]) const;
+
+private:
+    void compress_buffer();
+    void finish_chunk();
+    uint32_t cv_[8];
+    uint8_t block_[64];
+    uint64_t chunk_counter_;
+    uint32_t block_len_, blocks_compressed_;
+    uint32_t stack_[54][8];
+    uint32_t stack_len_;
+};
+
 // Fixed device workspace of the allocation-free entry point (pe_noalloc.hip).
 size_t scratch_bytes();
 hipError_t scratch_setup(int device);   // with `device` current; once per device
@@ -240,6 +275,9 @@ void* scratch_pointer(int device);
 // Launchers (pe_kernels_f64.hip / pe_kernels_f32.hip).
 hipError_t launch_bs_batch_f64(const BsBatchArgs& a, bool risk, bool cont, uint32_t graph_regs, hipStream_t s);
 hipError_t launch_bs_batch_f32(const BsBatchArgs& a, bool risk, bool cont, uint32_t graph_regs, hipStream_t s);
+// The same launch as a kernel node of `graph` (no dependencies).
+hipError_t add_bs_batch_node_f64(hipGraph_t graph, const BsBatchArgs& a, bool risk, bool cont, uint32_t graph_regs);
+hipError_t add_bs_batch_node_f32(hipGraph_t graph, const BsBatchArgs& a, bool risk, bool cont, uint32_t graph_regs);
 hipError_t launch_rough_f64(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 grid, uint32_t payoff_items,
                             unsigned long long* error, hipStream_t s);
 hipError_t launch_rough_f32(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 grid, uint32_t payoff_items,

@@ -322,6 +322,7 @@ struct LsmTrade {
     // fixed-policy Greeks (valuation kernel): Gamma bump, validation spot and vol bumps
     double gamma_bump, spot_bump, vol_bump;
     uint32_t risk, request_gamma;
+    uint64_t stop_offset;         // valuation trajectories before this trade (stopping-index array)
 };
 
 constexpr int kLsmRiskComponents = 9;   // Delta .. Gamma difference (PE component indices 1..9)
@@ -358,6 +359,7 @@ struct LsmArgs {
     void* risk_partials;              // kLsmRiskComponents BlockStats per valuation item (fixed-policy Greeks)
     unsigned long long* error;
     const LsmCuts* cuts;              // network policy: [trade][non-terminal date]
+    uint32_t* stops;                  // optional: exercise-date index per valuation trajectory
     uint32_t n_items, n_trades;
 };
 

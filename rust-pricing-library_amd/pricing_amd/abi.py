@@ -1,7 +1,7 @@
 """ctypes mirror of include/pe.h (the engine's C ABI)."""
 import ctypes as C
 
-PE_ABI_VERSION = 2
+PE_ABI_VERSION = 3
 
 # pe_status
 PE_OK = 0
@@ -164,6 +164,8 @@ class pe_risk(C.Structure):
         ("gamma_bump", C.c_double),
         ("smoothing", C.c_int32),
         ("smoothing_half_width", C.c_double),
+        ("aad_tile_capacity", C.c_uint32),
+        ("checkpoint_interval", C.c_uint32),
     ]
 
 
@@ -214,6 +216,38 @@ class pe_estimate(C.Structure):
     ]
 
 
+PE_ESTIMATOR_PSEUDO_MONTE_CARLO = 0
+PE_ESTIMATOR_RQMC = 1
+CURVE_REGIONS = ("pillar", "interpolated", "right_extrapolated")
+VALUATION_KINDS = ("exact_contractual", "smoothed_surrogate")
+PE_DOMAIN_VALUATION = 0
+PE_DOMAIN_LSM_TRAIN = 1
+PE_DOMAIN_RQMC_SCRAMBLE = 2
+RANDOM_DOMAINS = ("valuation", "lsm_train", "rqmc_scramble")
+
+
+class pe_mc_diagnostics(C.Structure):
+    _fields_ = [
+        ("master_seed", C.c_uint64),
+        ("estimator", C.c_int),
+        ("scramble_count", C.c_uint32),
+        ("has_checksums", C.c_int32),
+        ("direction_checksum", C.c_uint8 * 32),
+        ("scramble_checksum", C.c_uint8 * 32),
+        ("policy_version", C.c_uint32),
+        ("worker_threads", C.c_uint32),
+        ("reduction_block_size", C.c_uint64),
+        ("aad_tile_policy_version", C.c_uint32),
+        ("aad_tile_capacity", C.c_uint32),
+        ("checkpoint_policy_version", C.c_uint32),
+        ("checkpoint_interval", C.c_uint32),
+        ("antithetic", C.c_int32),
+        ("discount_region", C.c_int),
+        ("dividend_region", C.c_int),
+        ("valuation_kind", C.c_int),
+    ]
+
+
 class pe_result(C.Structure):
     _fields_ = [
         ("estimates", pe_estimate * COMPONENTS),
@@ -229,6 +263,7 @@ class pe_result(C.Structure):
         ("sobol_dimension", C.c_uint32),
         ("device_ms", C.c_double),
         ("failing_unit", C.c_uint64),
+        ("diagnostics", pe_mc_diagnostics),
     ]
 
 
@@ -316,6 +351,28 @@ class pe_lsm_report(C.Structure):
         ("exercise_counts", C.POINTER(C.c_uint64)),
         ("decisions", C.POINTER(pe_lsm_decision)),
         ("path_ms", C.c_double),
+        ("exercise_probabilities", _DP),
+        ("dividend_collisions", C.POINTER(C.c_int32)),
+        ("stopping_indices", C.POINTER(C.c_uint32)),
+        ("stopping_capacity", C.c_uint64),
+        ("training_random_domain", C.c_int),
+        ("valuation_random_domain", C.c_int),
+        ("training_sampling_units", C.c_uint64),
+        ("training_trajectories", C.c_uint64),
+        ("valuation_sampling_units", C.c_uint64),
+        ("valuation_trajectories", C.c_uint64),
+        ("training_has_checksums", C.c_int32),
+        ("valuation_has_checksums", C.c_int32),
+        ("training_direction_checksum", C.c_uint8 * 32),
+        ("training_scramble_checksum", C.c_uint8 * 32),
+        ("valuation_direction_checksum", C.c_uint8 * 32),
+        ("valuation_scramble_checksum", C.c_uint8 * 32),
+        ("policy_feature_count", C.c_uint32),
+        ("policy_max_degree", C.c_uint32),
+        ("itm_abs_tolerance", C.c_double),
+        ("abs_rank_tolerance", C.c_double),
+        ("rel_rank_tolerance", C.c_double),
+        ("max_matrix_elements", C.c_uint64),
     ]
 
 
@@ -345,6 +402,8 @@ PE_SYMBOLS = {
                                    C.POINTER(C.c_float), _DP]),
     "pe_mlp_forward": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint32, C.POINTER(C.c_float)]),
     "pe_last_kernel_stats": (C.c_int, [_DP, C.POINTER(C.c_uint32)]),
+    "pe_plan_diagnostics": (C.c_int, [C.c_void_p, C.POINTER(pe_mc_diagnostics)]),
+    "pe_blake3_256": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_uint8)]),
 }
 
 

@@ -40,6 +40,7 @@ SOURCES = {
     "pe_api.cpp": ["-ffp-contract=off"],
     "pe_surface.cpp": ["-ffp-contract=off"],
     "pe_graph.cpp": ["-ffp-contract=off"],
+    "pe_blake3.cpp": [],
 }
 
 

@@ -137,21 +137,46 @@ def evaluate_batch(plans, backend="fp64"):
 
 class LsmReport:
     """Early-exercise diagnostics of one plan: in-sample value, valuation
-    exercise counts, the decision model per non-terminal date, device ms."""
+    exercise counts and probabilities, the decision model per non-terminal
+    date, dividend collisions, optional stopping indices, the engines'
+    domains, sizes and RQMC checksums, device ms (EarlyExerciseDiagnostics)."""
 
-    def __init__(self, r, counts, decisions):
+    def __init__(self, r, counts, decisions, probabilities, collisions, stops):
+        n = r.exercise_dates
         self.in_sample_value = r.in_sample_value
         self.training_ms = r.training_ms
         self.valuation_ms = r.valuation_ms
         self.path_ms = r.path_ms
-        self.exercise_dates = r.exercise_dates
-        self.exercise_counts = list(counts)[:r.exercise_dates]
-        self.decisions = list(decisions)[:max(r.exercise_dates - 1, 0)]
+        self.exercise_dates = n
+        self.exercise_counts = list(counts)[:n]
+        self.exercise_probabilities = list(probabilities)[:n]
+        self.dividend_collisions = [bool(c) for c in list(collisions)[:n]]
+        self.decisions = list(decisions)[:max(n - 1, 0)]
+        self.stopping_indices = list(stops)[:r.valuation_trajectories] if stops is not None else None
+        self.training_random_domain = r.training_random_domain
+        self.valuation_random_domain = r.valuation_random_domain
+        self.training_sampling_units = r.training_sampling_units
+        self.training_trajectories = r.training_trajectories
+        self.valuation_sampling_units = r.valuation_sampling_units
+        self.valuation_trajectories = r.valuation_trajectories
+        self.training_direction_checksum = bytes(r.training_direction_checksum).hex() if r.training_has_checksums else None
+        self.training_scramble_checksum = bytes(r.training_scramble_checksum).hex() if r.training_has_checksums else None
+        self.valuation_direction_checksum = (bytes(r.valuation_direction_checksum).hex()
+                                             if r.valuation_has_checksums else None)
+        self.valuation_scramble_checksum = (bytes(r.valuation_scramble_checksum).hex()
+                                            if r.valuation_has_checksums else None)
+        self.policy_feature_count = r.policy_feature_count
+        self.policy_max_degree = r.policy_max_degree
+        self.itm_abs_tolerance = r.itm_abs_tolerance
+        self.abs_rank_tolerance = r.abs_rank_tolerance
+        self.rel_rank_tolerance = r.rel_rank_tolerance
+        self.max_matrix_elements = r.max_matrix_elements
 
 
-def evaluate_lsm(plans, backend="fp64", max_dates=1024):
+def evaluate_lsm(plans, backend="fp64", max_dates=1024, stopping_capacity=0):
     """Prices American plans by least-squares Monte Carlo on the device;
-    returns (results, reports)."""
+    returns (results, reports).  With stopping_capacity > 0 each report also
+    carries up to that many stopping indices (one per valuation trajectory)."""
     n = len(plans)
     handles = (C.c_void_p * n)(*[p._handle.value for p in plans])
     results = (abi.pe_result * n)()
@@ -160,12 +185,20 @@ def evaluate_lsm(plans, backend="fp64", max_dates=1024):
     for r in reports:
         counts = (C.c_uint64 * max_dates)()
         decisions = (abi.pe_lsm_decision * max_dates)()
+        probabilities = (C.c_double * max_dates)()
+        collisions = (C.c_int32 * max_dates)()
+        stops = (C.c_uint32 * stopping_capacity)() if stopping_capacity else None
         r.capacity = max_dates
         r.exercise_counts = C.cast(counts, C.POINTER(C.c_uint64))
         r.decisions = C.cast(decisions, C.POINTER(abi.pe_lsm_decision))
-        keep.append((counts, decisions))
+        r.exercise_probabilities = C.cast(probabilities, C.POINTER(C.c_double))
+        r.dividend_collisions = C.cast(collisions, C.POINTER(C.c_int32))
+        if stops is not None:
+            r.stopping_indices = C.cast(stops, C.POINTER(C.c_uint32))
+            r.stopping_capacity = stopping_capacity
+        keep.append((counts, decisions, probabilities, collisions, stops))
     _check(load().pe_lsm_evaluate(handles, n, BACKENDS[backend], results, reports))
-    return list(results), [LsmReport(r, c, d) for r, (c, d) in zip(reports, keep)]
+    return list(results), [LsmReport(r, *k) for r, k in zip(reports, keep)]
 
 
 def evaluate_lsm_dual(plan, continuation_inner_paths, exercise_inner_paths, inner_seed, backend="fp64"):

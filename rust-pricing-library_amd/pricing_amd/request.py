@@ -232,8 +232,10 @@ class Request:
         return self
 
     def set_risk(self, delta=False, vega=False, gamma_bump=None, gamma_relative=True,
-                 smoothing_half_width=None):
+                 smoothing_half_width=None, aad_tile_capacity=None, checkpoint_interval=None):
         r = self.c.risk
+        r.aad_tile_capacity = int(aad_tile_capacity or 0)
+        r.checkpoint_interval = int(checkpoint_interval or 0)
         r.delta = int(bool(delta))
         r.vega = int(bool(vega))
         r.gamma = 0 if gamma_bump is None else 1
@@ -349,7 +351,8 @@ class Request:
             gamma_relative = gamma["bump"]["type"] == "relative"
         smoothing = risk.get("payoff_smoothing")
         self.set_risk(risk.get("delta", False), risk.get("vega", False), gamma_bump, gamma_relative,
-                      smoothing["half_width"] if smoothing else None)
+                      smoothing["half_width"] if smoothing else None, risk.get("aad_tile_capacity"),
+                      risk.get("checkpoint_interval"))
         return self
 
 

@@ -10,7 +10,9 @@ from
   fixtures/path-dependence/reference-cases-v0.1.json  (smoothing, extrema, bridge, smoothed bridge, jumps)
   fixtures/rough-volatility/lifted-factor-prices.json  (finite lift kernels)
   fixtures/rough-volatility/two-step-prices.json  (lifted Heston two-step call prices)
-  fixtures/replay/early_exercise-windows-x86_64.json  (LSM requests, prices, fitted decision models)
+  fixtures/replay/early_exercise-windows-x86_64.json  (LSM requests, prices, fitted decision models,
+      early-exercise diagnostics)
+  (replay cases also keep the execution's monte_carlo diagnostics block)
   fixtures/early-exercise/reference-cases-v0.1.json  (decision, basis, scaling, QR cases)
   fixtures/v3/*.golden.json  (schema-v3 request/result documents)
   fixtures/product-graph-v1|v2/*.json  (copied to product_graphs/)
@@ -51,6 +53,7 @@ def replay(name):
             "barrier_bridge": (case.get("path_diagnostics") or {}).get("barrier_bridge"),
             "request_fingerprint": result["replay"]["request_fingerprint"],
             "result_risks": {k: v for k, v in (result.get("risks") or {}).items() if isinstance(v, dict) and "raw" in v},
+            "monte_carlo": case["execution"]["monte_carlo"],
         })
     return cases
 
@@ -89,7 +92,9 @@ def main():
         "risks": {k: {"value": v["raw"]["value"], "standard_error": v["raw"]["standard_error"]}
                   for k, v in (c["result"].get("risks") or {}).items() if isinstance(v, dict) and "raw" in v},
         "early_exercise": {k: v for k, v in c["result"]["monte_carlo"]["early_exercise"].items()
-                           if k in ("decision_models", "exercise_counts", "in_sample_value", "stopping_indices")},
+                           if k != "regression_diagnostics"},
+        "monte_carlo": c["execution"]["monte_carlo"],
+        "worker_threads": c["plan"]["worker_threads"],
     } for c in ee["cases"]]
     with open(os.path.join(ROOT, "fixtures", "early-exercise", "reference-cases-v0.1.json")) as f:
         out["early_exercise_cases"] = json.load(f)

@@ -1,8 +1,12 @@
 """CPU-side checks of the C ABI: the library loads and exports every entry
-point include/pe.h declares, and the ctypes mirror matches the header."""
+point include/pe.h declares, and the ctypes mirror matches the header: the
+size and every field offset of every struct, as a C compiler lays them out
+from pe.h."""
 import ctypes as C
 import os
 import re
+import subprocess
+import tempfile
 
 from pricing_amd import abi, engine
 
@@ -28,7 +32,35 @@ def test_library_exports_every_declared_symbol():
     assert lib.pe_abi_version() == abi.PE_ABI_VERSION
 
 
-def test_struct_sizes_match_c_layout():
-    # pe_result: 18 estimates (4 doubles + u64) + 3x18 doubles + 2 doubles + 3 u64 + 2 u32 + double + u64
-    assert C.sizeof(abi.pe_estimate) == 40
-    assert C.sizeof(abi.pe_result) == 18 * 40 + 3 * 18 * 8 + 2 * 8 + 3 * 8 + 2 * 4 + 8 + 8
+def _mirrors():
+    return {name: cls for name, cls in vars(abi).items()
+            if name.startswith("pe_") and isinstance(cls, type) and issubclass(cls, C.Structure)}
+
+
+def test_every_header_struct_has_a_ctypes_mirror():
+    text = open(HEADER).read()
+    declared = set(re.findall(r"typedef struct (pe_[a-z0-9_]+) \{", text))
+    assert declared and declared == set(_mirrors()), declared ^ set(_mirrors())
+
+
+def test_struct_layout_matches_the_c_compiler():
+    """Compiles a probe against pe.h with the host C compiler and compares
+    sizeof / offsetof of every field with the ctypes mirror."""
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "pe.h"', "int main(void) {"]
+    expected = []
+    for name, cls in sorted(_mirrors().items()):
+        lines.append(f'printf("{name} %zu\\n", sizeof({name}));')
+        expected.append(f"{name} {C.sizeof(cls)}")
+        for field in cls._fields_:
+            f = field[0]
+            lines.append(f'printf("{name}.{f} %zu\\n", offsetof({name}, {f}));')
+            expected.append(f"{name}.{f} {getattr(cls, f).offset}")
+    lines += ["return 0;", "}"]
+    with tempfile.TemporaryDirectory() as tmp:
+        src = os.path.join(tmp, "layout.c")
+        exe = os.path.join(tmp, "layout")
+        with open(src, "w") as fh:
+            fh.write("\n".join(lines))
+        subprocess.check_call(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.dirname(HEADER), src, "-o", exe])
+        got = subprocess.check_output([exe], text=True).split("\n")[:-1]
+    assert got == expected, [(g, e) for g, e in zip(got, expected) if g != e]

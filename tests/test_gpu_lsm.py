@@ -27,9 +27,9 @@ def _init():
 REPLAY = golden()["early_exercise_replay"]
 
 
-def _device(req, backend="fp64"):
+def _device(req, backend="fp64", stopping_capacity=0):
     plan = engine.PricingPlan.compile(req)
-    res, rep = engine.evaluate_lsm([plan], backend)
+    res, rep = engine.evaluate_lsm([plan], backend, stopping_capacity=stopping_capacity)
     return res[0], rep[0]
 
 
@@ -79,7 +79,7 @@ def _compare_oracle(req, block=256):
 @pytest.mark.parametrize("case", REPLAY, ids=[c["name"] for c in REPLAY])
 def test_replay_on_device(case):
     req = request.Request.from_v3(case["request"])
-    dev, rep = _device(req)
+    dev, rep = _device(req, stopping_capacity=len(case["early_exercise"]["stopping_indices"]))
     exp = case["value"]
     # fixed-policy Greeks: pathwise at each path's stopping date; Gamma is a
     # difference of pathwise Deltas over 2h (rounding scale |Delta| / h)
@@ -94,6 +94,27 @@ def test_replay_on_device(case):
     assert abs(dev.estimates[0].standard_error - exp["standard_error"]) <= 1e-10 * exp["standard_error"]
     ee = case["early_exercise"]
     assert rep.exercise_counts[:len(ee["exercise_counts"])] == ee["exercise_counts"]
+    # EarlyExerciseDiagnostics: probabilities, stopping index of every
+    # valuation trajectory, collisions, engine sizes, domains and checksums
+    assert rep.exercise_probabilities == ee["exercise_probabilities"]
+    assert rep.dividend_collisions == ee["dividend_collisions"]
+    assert rep.stopping_indices == ee["stopping_indices"]
+    assert rep.training_sampling_units == ee["training_sampling_units"]
+    assert rep.training_trajectories == ee["training_trajectories"]
+    assert rep.valuation_sampling_units == ee["valuation_sampling_units"]
+    assert rep.valuation_trajectories == ee["valuation_trajectories"]
+    assert abi.RANDOM_DOMAINS[rep.training_random_domain] == ee["training_random_domain"]["type"]
+    assert abi.RANDOM_DOMAINS[rep.valuation_random_domain] == ee["valuation_random_domain"]["type"]
+    for k in ("training_direction_checksum", "training_scramble_checksum", "valuation_direction_checksum",
+              "valuation_scramble_checksum"):
+        assert getattr(rep, k) == ee.get(k), k
+    assert rep.policy_max_degree == ee["policy_basis"]["max_degree"]
+    assert rep.policy_feature_count == ee["policy_basis"]["feature_count"]
+    assert rep.itm_abs_tolerance == ee["itm_abs_tolerance"] and rep.max_matrix_elements == ee["max_matrix_elements"]
+    assert rep.abs_rank_tolerance == ee["cpqr"]["abs_rank_tolerance"]
+    assert rep.rel_rank_tolerance == ee["cpqr"]["rel_rank_tolerance"]
+    mc = dev.diagnostics
+    assert mc.master_seed == int(case["monte_carlo"]["master_seed"]) and mc.policy_version == 1
     assert abs(rep.in_sample_value - ee["in_sample_value"]) <= 1e-12 * abs(ee["in_sample_value"])
     for d, model in enumerate(ee["decision_models"]):
         m = model["model"]

@@ -86,6 +86,12 @@ def test_replay_fixture_on_device(case):
         if key in case["risks"]:
             e = case["risks"][key]
             assert close(res.estimates[idx].value, e["value"]), (key, res.estimates[idx].value, e["value"])
+    # the evaluation reports the fixture's RQMC checksums and engine identity
+    mc, d = case["monte_carlo"], res.diagnostics
+    assert d.master_seed == int(mc["master_seed"]) and d.scramble_count == (mc["scramble_count"] or 0)
+    if mc["direction_checksum"]:
+        assert bytes(d.direction_checksum).hex() == mc["direction_checksum"]
+        assert bytes(d.scramble_checksum).hex() == mc["scramble_checksum"]
 
 
 def test_rqmc_points_not_multiple_of_wave():
