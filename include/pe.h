@@ -604,6 +604,13 @@ pe_status pe_local_variance_from_ssvi(const pe_ssvi_surface* surface,
                                       double floor, double cap,
                                       double* values_out, uint32_t* repair_count_out);
 
+/* Accuracy of the mixed mode's FP32 inverse normal on the device: the largest
+ * relative error of z against the FP64 AS241 over words [first_word,
+ * first_word + count) and a word attaining it (verification, no reference
+ * counterpart). */
+pe_status pe_normal_mixed_max_error(uint64_t first_word, uint64_t count, double* max_relative_error,
+                                    uint32_t* worst_word);
+
 /* Kernel-time instrumentation of the last evaluate on this thread: average
  * duration of the dominant kernel over its launches, in milliseconds. */
 pe_status pe_last_kernel_stats(double* avg_ms, uint32_t* launches);

@@ -1843,6 +1843,29 @@ pe_status pe_mlp_forward(const float* params, const float* x, uint32_t rows, flo
     });
 }
 
+pe_status pe_normal_mixed_max_error(uint64_t first_word, uint64_t count, double* max_relative_error,
+                                    uint32_t* worst_word) {
+    if (!max_relative_error || count == 0 || first_word + count > (1ull << 32))
+        return fail(PE_ERR_INVALID_ARGUMENT, "word range");
+    return guarded([&] {
+        ensure_init();
+        CallStream cs;
+        constexpr uint32_t kBlocks = 4096;
+        DeviceBuffer err, word;
+        err.reserve(kBlocks * sizeof(double));
+        word.reserve(kBlocks * sizeof(uint32_t));
+        check(launch_normal_mixed_error(first_word, count, kBlocks, err.as<double>(), word.as<uint32_t>(), t_stream),
+              "accuracy kernel launch");
+        std::vector<double> e(kBlocks);
+        std::vector<uint32_t> w(kBlocks);
+        check(copy_on_stream(e.data(), err.ptr, kBlocks * sizeof(double), hipMemcpyDeviceToHost), "copy");
+        check(copy_on_stream(w.data(), word.ptr, kBlocks * sizeof(uint32_t), hipMemcpyDeviceToHost), "copy");
+        size_t k = static_cast<size_t>(std::max_element(e.begin(), e.end()) - e.begin());
+        *max_relative_error = e[k];
+        if (worst_word) *worst_word = w[k];
+    });
+}
+
 pe_status pe_last_kernel_stats(double* avg_ms, uint32_t* launches) {
     if (avg_ms) *avg_ms = g_kernel_ms;
     if (launches) *launches = g_kernel_launches;

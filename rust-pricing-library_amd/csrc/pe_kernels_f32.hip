@@ -15,6 +15,46 @@
 
 namespace pe {
 
+// Accuracy of the mixed inverse normal against the FP64 AS241 over a range of
+// words: per block the largest relative error of z and its word.
+__global__ void __launch_bounds__(kBlockThreads) normal_mixed_error_kernel(uint64_t first, uint64_t count,
+                                                                           double* block_err, uint32_t* block_word) {
+    __shared__ double err[kBlockThreads];
+    __shared__ uint32_t at[kBlockThreads];
+    double worst = 0.0;
+    uint32_t worst_word = 0;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const uint32_t w = static_cast<uint32_t>(first + i);
+        const double z = static_cast<double>(normal_from_word_mixed(w));
+        const double ref = normal_from_word_f64(w);
+        const double e = fabs(z - ref) / fabs(ref);
+        if (e > worst || !(e == e)) {
+            worst = e == e ? e : 1e300;
+            worst_word = w;
+        }
+    }
+    err[threadIdx.x] = worst;
+    at[threadIdx.x] = worst_word;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBlockThreads; ++k)
+            if (err[k] > worst) {
+                worst = err[k];
+                worst_word = at[k];
+            }
+        block_err[blockIdx.x] = worst;
+        block_word[blockIdx.x] = worst_word;
+    }
+}
+
+hipError_t launch_normal_mixed_error(uint64_t first, uint64_t count, uint32_t blocks, double* block_err,
+                                     uint32_t* block_word, hipStream_t s) {
+    hipLaunchKernelGGL(normal_mixed_error_kernel, dim3(blocks), dim3(kBlockThreads), 0, s, first, count, block_err,
+                       block_word);
+    return hipGetLastError();
+}
+
 hipError_t add_bs_batch_node_f32(hipGraph_t graph, const BsBatchArgs& a, bool risk, bool cont, uint32_t graph_regs) {
     return add_bs_batch_node<float>(graph, a, risk, cont, graph_regs);
 }

@@ -298,6 +298,8 @@ hipError_t launch_mlp_gradient(const float* params, const float* x, const float*
                                uint32_t wgs, float* partials, hipStream_t s);
 hipError_t launch_mlp_forward(const float* params, const float* x, uint32_t rows, float* out, hipStream_t s);
 constexpr uint32_t kMlpParameterCount = 4417;
+hipError_t launch_normal_mixed_error(uint64_t first, uint64_t count, uint32_t blocks, double* block_err,
+                                     uint32_t* block_word, hipStream_t s);
 hipError_t launch_lv_f64(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s);
 hipError_t launch_lv_f32(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s);
 hipError_t launch_dupire(const double* theta_t, const double* theta_v, const double* theta_d, uint32_t nk,

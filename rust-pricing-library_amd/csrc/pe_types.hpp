@@ -35,7 +35,7 @@ enum GraphCode : uint16_t {
     kGMulAdd   // a * b + c: a Multiply fused into the Add that is its only consumer
 };
 
-struct __align__(64) GraphOp {   // one 64-byte scalar load per operation
+struct alignas(64) GraphOp {   // one 64-byte scalar load per operation
     uint32_t code, out;      // GraphCode, destination register
     uint32_t a, b, c;        // operands (register, or kGraphConst: value in ka/kb/kc)
     uint32_t pad;

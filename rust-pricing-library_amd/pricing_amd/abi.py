@@ -403,6 +403,7 @@ PE_SYMBOLS = {
     "pe_mlp_forward": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint32, C.POINTER(C.c_float)]),
     "pe_last_kernel_stats": (C.c_int, [_DP, C.POINTER(C.c_uint32)]),
     "pe_plan_diagnostics": (C.c_int, [C.c_void_p, C.POINTER(pe_mc_diagnostics)]),
+    "pe_normal_mixed_max_error": (C.c_int, [C.c_uint64, C.c_uint64, _DP, C.POINTER(C.c_uint32)]),
     "pe_blake3_256": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_uint8)]),
 }
 

@@ -83,6 +83,49 @@ def build(verbose=False):
     return LIB
 
 
+SANITIZE = ["-Xarch_host", "-fsanitize=address,undefined", "-Xarch_host", "-fno-sanitize-recover=all",
+            "-Xarch_host", "-fno-omit-frame-pointer", "-g"]
+LIB_ASAN = os.path.join(LIB_DIR, "librpl_amd_asan.so")
+
+
+def build_sanitized(verbose=False):
+    """The engine with AddressSanitizer + UndefinedBehaviorSanitizer on its
+    host code (plan compiler, graph compiler, C ABI, BLAKE3, surface); the
+    device kernels are built as usual.  Load it with the clang ASan runtime
+    preloaded (see sanitizer_runtime())."""
+    build()   # the ordinary objects, reused for the HIP translation units
+    objs = []
+    for src, extra in SOURCES.items():
+        if src.endswith(".hip"):
+            objs.append(os.path.join(OBJ_DIR, src + ".o"))
+            continue
+        out = os.path.join(OBJ_DIR, src + ".asan.o")
+        _run([HIPCC] + COMMON + extra + SANITIZE + ["-c", os.path.join(CSRC, src), "-o", out])
+        objs.append(out)
+    objs.append(os.path.join(OBJ_DIR, "pe_directions.o"))
+    tmp = LIB_ASAN + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-shared-libasan", "-Xarch_host",
+          "-fsanitize=address,undefined", "-o", tmp] + objs)
+    os.replace(tmp, LIB_ASAN)
+    if verbose:
+        print(LIB_ASAN)
+    return LIB_ASAN
+
+
+def sanitizer_runtime():
+    """The clang ASan runtime the sanitized libraries link against."""
+    import glob
+    root = os.path.dirname(os.path.dirname(os.path.realpath(HIPCC)))
+    hits = sorted(glob.glob(os.path.join(root, "llvm", "lib", "clang", "*", "lib", "linux",
+                                         "libclang_rt.asan-x86_64.so")))
+    if not hits:
+        hits = sorted(glob.glob("/opt/rocm/llvm/lib/clang/*/lib/linux/libclang_rt.asan-x86_64.so"))
+    return hits[-1] if hits else None
+
+
 if __name__ == "__main__":
-    build(verbose=True)
+    if "--sanitize" in sys.argv:
+        build_sanitized(verbose=True)
+    else:
+        build(verbose=True)
     sys.exit(0)

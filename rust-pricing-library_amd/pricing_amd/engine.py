@@ -10,7 +10,8 @@ import os
 
 from . import abi
 
-_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "librpl_amd.so")
+_LIB_PATH = os.environ.get("PE_ENGINE_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib",
+                                                             "librpl_amd.so")
 _lib = None
 
 BACKENDS = {"fp64": abi.PE_BACKEND_GPU_FP64, "mixed": abi.PE_BACKEND_GPU_MIXED}

@@ -150,12 +150,58 @@ def encode_value(value, out=None):
     return out
 
 
+# Fields the v3 request schema types as JSON numbers (schemas/v3/pricing_request.schema.json):
+# RequestV3 holds them as f64, so its re-serialisation -- what fingerprint_request
+# hashes (wire.rs:4144-4161) -- writes them as floats even when the input wrote an
+# integer.  Every other number in the schema is an integer field.  No name is
+# typed both ways, so the field name decides.
+_FLOAT_FIELDS = frozenset({
+    "abs_rank_tolerance", "amount", "barrier", "beta", "cap", "cdf_tolerance", "discount_factors", "ex_time",
+    "fixed_cash", "fixing", "floor", "grid_width", "half_width", "historical_extremum", "implied_volatilities",
+    "itm_abs_tolerance", "iv_bump", "log_forward_moneyness_nodes", "maturity_nodes", "notional",
+    "payoff_smoothing_width_ladder", "payout", "projection_mean_tolerance", "projection_nodes",
+    "projection_tail_tolerance", "rebate", "rel_rank_tolerance", "relative_density_threshold", "spot", "strike",
+    "tail_tolerance", "time_nodes", "times", "valuation_date_fixing", "value", "values", "volatility", "weight",
+})
+
+# Product variants of the v3 request schema; a graph product has no v3
+# request form (its fingerprint uses another domain) and is refused.
+_V3_PRODUCTS = frozenset({"european_vanilla", "american_vanilla", "digital", "barrier", "arithmetic_asian",
+                          "fixed_lookback"})
+
+
+def _as_float(value):
+    if isinstance(value, bool):
+        return value
+    if isinstance(value, int):
+        return float(value)
+    if isinstance(value, list):
+        return [_as_float(v) for v in value]
+    return value
+
+
+def canonical_request(value, key=None):
+    """The request document as RequestV3 re-serialises it: numbers of
+    float-typed fields as floats (an integer 100 becomes 100.0)."""
+    if isinstance(value, dict):
+        return {k: canonical_request(v, k) for k, v in value.items()}
+    if isinstance(value, list):
+        return [canonical_request(v, key) for v in value]
+    if key in _FLOAT_FIELDS:
+        return _as_float(value)
+    return value
+
+
 def request_fingerprint(doc):
-    """fingerprint_request for a schema-v3 pricing_request document."""
+    """fingerprint_request for a schema-v3 pricing_request document, hashed
+    in its canonical form (canonical_request)."""
     if doc.get("document_kind") != "pricing_request":
         raise ValueError("not a pricing_request document")
+    product = doc.get("product", {}).get("type")
+    if product not in _V3_PRODUCTS:
+        raise ValueError(f"product {product!r} has no schema-v3 request fingerprint")
     data = bytearray(b"pricing/request\0") + struct.pack(">I", int(doc.get("schema_version", 3)))
-    encode_value(doc, data)
+    encode_value(canonical_request(doc), data)
     return "blake3-256:" + blake3_256(bytes(data))
 
 

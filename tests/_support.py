@@ -30,7 +30,7 @@ def oracle():
         subprocess.check_call(["make", "-s", "-C", ORACLE_DIR])
     from pricing_amd import abi
 
-    lib = C.CDLL(ORACLE_SO)
+    lib = C.CDLL(os.environ.get("PE_ORACLE_LIB") or ORACLE_SO)
     lib.oracle_init.argtypes = [C.c_char_p]
     lib.oracle_last_error.restype = C.c_char_p
     lib.oracle_philox.argtypes = [C.c_uint64, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]

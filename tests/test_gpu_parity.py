@@ -172,3 +172,15 @@ def test_local_vol_pseudo_bridge_and_step_order_match_oracle():
         req.c.engine.brownian_bridge = bridge
         req.c.engine.units = 3000
         compare(engine.PricingPlan.compile(req).evaluate("fp64"), oracle_evaluate(req, workers=8), [0])
+
+
+def test_normal_mixed_device_accuracy():
+    """The mixed kernels' FP32 inverse normal with the device's native log2
+    and sqrt, over every 32-bit word, against the FP64 AS241 of the same
+    midpoint probability: relative error of z below 7e-7."""
+    import ctypes as C
+
+    err, word = C.c_double(), C.c_uint32()
+    assert engine.load().pe_normal_mixed_max_error(0, 1 << 32, C.byref(err), C.byref(word)) == abi.PE_OK
+    print("max relative error %.3e at word %d" % (err.value, word.value), flush=True)
+    assert err.value < 7e-7, (err.value, word.value)

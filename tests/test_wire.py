@@ -10,7 +10,8 @@ import pytest
 from _support import golden, oracle_evaluate
 from pricing_amd import request, wire
 
-CASES = golden()["replay"]
+G = golden()
+CASES = G["replay"]
 
 
 def test_blake3_empty_input():
@@ -86,3 +87,29 @@ def test_v3_golden_american_documents_pair_up():
     replay = v["pricing_result_american.golden.json"]["replay"]
     assert fp == replay["request_fingerprint"]
     assert fp == replay["migration"]["pre_migration_fingerprint"] == replay["migration"]["post_migration_fingerprint"]
+
+
+def test_integer_written_floats_keep_the_fingerprint():
+    """RequestV3 parses 100 and 100.0 into the same f64: the fingerprint of
+    the canonical re-serialisation does not change (golden American request)."""
+    import copy
+
+    doc = G["v3_documents"]["pricing_request_american.golden.json"]
+    want = wire.request_fingerprint(doc)
+    alt = copy.deepcopy(doc)
+    alt["market"]["spot"] = int(alt["market"]["spot"])
+    alt["product"]["strike"] = int(alt["product"]["strike"])
+    assert isinstance(alt["market"]["spot"], int)
+    assert wire.request_fingerprint(alt) == want
+    # an integer field stays an integer: changing its value changes the hash
+    alt["engine"]["master_seed"] = int(alt["engine"]["master_seed"]) + 1
+    assert wire.request_fingerprint(alt) != want
+
+
+def test_graph_products_are_refused():
+    import copy
+
+    doc = copy.deepcopy(G["v3_documents"]["pricing_request.golden.json"])
+    doc["product"] = {"type": "graph"}
+    with pytest.raises(ValueError):
+        wire.request_fingerprint(doc)
