@@ -464,6 +464,7 @@ typedef struct pe_lsm_decision {
     int32_t region_left;
     uint32_t region_cut_count;
     float region_cuts[64];
+    double zero_scale_threshold;    /* FeatureScaling: 64 eps max(1, max |ITM spot|) (monomial basis) */
 } pe_lsm_decision;
 
 /* Early-exercise diagnostics of one plan (EarlyExerciseDiagnostics,

@@ -1584,6 +1584,7 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
             o.feature_inactive = s.inactive ? 1 : 0;
             o.rank_threshold = s.rank_threshold;
             o.residual_sum_squares = s.rss;
+            o.zero_scale_threshold = s.zero_threshold;
         }
     }
     g_kernel_ms = train_ms + value_ms;

@@ -333,6 +333,7 @@ struct LsmDecision {
     double diag[kLsmMaxBasis];    // |R_pp| in pivot order
     double mean, variance, scale, inv_scale;   // monomial feature standardisation
     double rank_threshold, rss;
+    double zero_threshold;        // FeatureScaling zero-scale threshold: 64 eps max(1, max |x|)
     uint32_t pivot[kLsmMaxBasis];
     uint32_t rank, continue_all, inactive, itm_rows;
     float fcoef[kLsmMaxBasis];    // FP32 copies for the mixed decisions

@@ -325,6 +325,7 @@ class pe_lsm_decision(C.Structure):
         ("region_left", C.c_int32),
         ("region_cut_count", C.c_uint32),
         ("region_cuts", C.c_float * 64),
+        ("zero_scale_threshold", C.c_double),
     ]
 
 

@@ -17,7 +17,8 @@ function, 1 KiB chunks and the binary chaining-value tree; unkeyed hash,
 """
 import struct
 
-__all__ = ["blake3_256", "encode_value", "request_fingerprint", "estimate_document", "result_document"]
+__all__ = ["blake3_256", "encode_value", "canonical_request", "request_fingerprint", "estimate_document",
+           "monte_carlo_diagnostics_document", "early_exercise_document", "result_document"]
 
 _IV = (0x6A09E667, 0xBB67AE85, 0x3C6EF372, 0xA54FF53A, 0x510E527F, 0x9B05688C, 0x1F83D9AB, 0x5BE0CD19)
 _PERMUTATION = (2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8)
@@ -215,9 +216,111 @@ def estimate_document(value, standard_error, effective_units, estimator, scale=1
             "estimator": {"type": estimator}, "effective_sampling_units": int(effective_units)}
 
 
-def result_document(doc, result, library_version="0.1.0", platform="mi355x-hip"):
+_REGIONS = ("pillar", "interpolated", "right_extrapolated")
+_VALUATION_KINDS = ("exact_contractual", "smoothed_surrogate")
+_DOMAINS = ("valuation", "lsm_train", "rqmc_scramble")
+
+
+def monte_carlo_diagnostics_document(d):
+    """MonteCarloDiagnostics (mc_result.rs:226-247) of a pe_result as the
+    result document writes it; the payoff-graph fingerprint is not carried."""
+    rqmc = d.estimator == 1
+    out = {
+        "aad_tile_capacity": int(d.aad_tile_capacity), "aad_tile_policy_version": int(d.aad_tile_policy_version),
+        "antithetic": bool(d.antithetic), "checkpoint_interval": int(d.checkpoint_interval),
+        "checkpoint_policy_version": int(d.checkpoint_policy_version),
+        "discount_region": {"type": _REGIONS[d.discount_region]},
+        "dividend_region": {"type": _REGIONS[d.dividend_region]},
+        "estimator": {"type": "randomized_quasi_monte_carlo" if rqmc else "pseudo_monte_carlo"},
+        "master_seed": int(d.master_seed), "policy_version": int(d.policy_version),
+        "reduction_block_size": int(d.reduction_block_size),
+        "valuation_kind": {"type": _VALUATION_KINDS[d.valuation_kind]}, "worker_threads": int(d.worker_threads),
+    }
+    if rqmc:
+        out["scramble_count"] = int(d.scramble_count)
+        out["direction_checksum"] = bytes(d.direction_checksum).hex()
+        out["scramble_checksum"] = bytes(d.scramble_checksum).hex()
+    return out
+
+
+def _decision_document(m, columns, max_degree):
+    """ExerciseDecisionModel of one date (wire.rs:2048-2090): a polynomial
+    regression on the standardized spot, or continue-all without in-the-money
+    training rows."""
+    if m.continue_all:
+        return {"type": "continue_all", "reason": {"type": "zero_itm_training_paths"}}, [
+            {"type": "zero_itm_training_paths"}]
+    pre_excluded = list(range(1, columns)) if m.feature_inactive else []
+    active = [c for c in range(columns) if c not in pre_excluded]
+    pivot = [int(v) for v in list(m.pivot_order)[:len(active)]]
+    rank = int(m.rank)
+    model = {
+        "active_basis_columns": active,
+        "basis": {"exponents": [[k] for k in range(columns)], "feature_count": 1, "max_degree": max_degree},
+        "coefficients": [float(v) for v in list(m.coefficients)[:columns]],
+        "diagonal_abs": [float(v) for v in list(m.diagonal_abs)[:len(active)]],
+        "feature_scalings": [{"inactive": bool(m.feature_inactive), "mean": m.feature_mean,
+                              "population_variance": m.feature_population_variance, "scale": m.feature_scale,
+                              "zero_scale_threshold": m.zero_scale_threshold}],
+        "pivot_order": pivot,
+        "pre_excluded_basis_columns": pre_excluded,
+        "rank": rank,
+        "rank_excluded_basis_columns": pivot[rank:],
+        "rank_threshold": m.rank_threshold,
+        "residual_sum_squares": m.residual_sum_squares,
+    }
+    warnings = ([{"type": "inactive_feature", "feature": 0}] if m.feature_inactive else []) + [
+        {"type": "rank_excluded_basis_column", "column": c} for c in pivot[rank:]]
+    return {"type": "regression", "model": model}, warnings
+
+
+def early_exercise_document(doc, report):
+    """EarlyExerciseDiagnostics (mc_result.rs:249-273) of an American request
+    from the device's pe_lsm_report (engine.LsmReport); the exercise-policy
+    fingerprint is not carried."""
+    lsm = doc["lsm"]
+    degree = int(lsm["basis"]["max_degree"])
+    columns = degree + 1
+    models, diagnostics = [], []
+    for m in report.decisions:
+        model, warnings = _decision_document(m, columns, degree)
+        models.append(model)
+        diagnostics.append({"candidate_rows": int(report.training_trajectories), "feature_count": 1,
+                            "itm_rows": int(m.itm_rows), "warnings": warnings})
+    out = {
+        "cpqr": {"abs_rank_tolerance": report.abs_rank_tolerance, "rel_rank_tolerance": report.rel_rank_tolerance},
+        "decision_models": models,
+        "dividend_collisions": list(report.dividend_collisions),
+        "exercise_counts": [int(c) for c in report.exercise_counts],
+        "exercise_dates": list(doc["product"]["exercise_dates"]),
+        "exercise_probabilities": list(report.exercise_probabilities),
+        "in_sample_value": report.in_sample_value,
+        "itm_abs_tolerance": report.itm_abs_tolerance,
+        "max_matrix_elements": int(report.max_matrix_elements),
+        "policy_basis": {"exponents": [[k] for k in range(columns)], "feature_count": 1, "max_degree": degree},
+        "regression_diagnostics": diagnostics,
+        "training_random_domain": {"type": _DOMAINS[report.training_random_domain]},
+        "training_sampling_units": int(report.training_sampling_units),
+        "training_trajectories": int(report.training_trajectories),
+        "valuation_random_domain": {"type": _DOMAINS[report.valuation_random_domain]},
+        "valuation_sampling_units": int(report.valuation_sampling_units),
+        "valuation_trajectories": int(report.valuation_trajectories),
+    }
+    if report.stopping_indices is not None:
+        out["stopping_indices"] = [int(v) for v in report.stopping_indices]
+    for k in ("training_direction_checksum", "training_scramble_checksum", "valuation_direction_checksum",
+              "valuation_scramble_checksum"):
+        if getattr(report, k) is not None:
+            out[k] = getattr(report, k)
+    return out
+
+
+def result_document(doc, result, library_version="0.1.0", platform="mi355x-hip", report=None):
     """pricing_result document for the request document `doc` priced into the
-    pe_result `result` (price and the Greeks the request asks for)."""
+    pe_result `result` (price and the Greeks the request asks for).  An
+    American request with its pe_lsm_report (engine.LsmReport) also gets the
+    monte_carlo block: the engine diagnostics and the early-exercise
+    diagnostics."""
     engine = doc["engine"]["type"]
     estimator = "pseudo_monte_carlo" if engine == "pseudo_monte_carlo" else "randomized_quasi_monte_carlo"
     spot = float(doc["market"]["spot"])
@@ -246,7 +349,7 @@ def result_document(doc, result, library_version="0.1.0", platform="mi355x-hip")
         risks["vega"] = {"raw": est(2), "raw_unit": {"type": "vega_raw"},
                          "market_scaled": est(2, 0.01), "market_scaled_unit": {"type": "vega_one_vol_point"}}
     fingerprint = request_fingerprint(doc)
-    return {
+    out = {
         "document_kind": "pricing_result",
         "schema_version": 3,
         "value": est(0),
@@ -261,3 +364,7 @@ def result_document(doc, result, library_version="0.1.0", platform="mi355x-hip")
                           "pre_migration_fingerprint": fingerprint, "post_migration_fingerprint": fingerprint},
         },
     }
+    if doc["product"]["type"] == "american_vanilla" and report is not None:
+        out["monte_carlo"] = {"diagnostics": monte_carlo_diagnostics_document(result.diagnostics),
+                              "early_exercise": early_exercise_document(doc, report)}
+    return out
