@@ -1229,7 +1229,14 @@ LsmLaunch& prepare_lsm(std::vector<HostPlan*>& plans, pe_backend backend) {
     auto L = std::make_unique<LsmLaunch>();
     L->key = key;
     L->backend = static_cast<int>(backend);
-    const size_t real = backend == PE_BACKEND_GPU_FP64 ? sizeof(double) : sizeof(float);
+    // Training paths are FP64 in both modes for the polynomial policies: the
+    // backward induction turns FP32 rounding of the stored spots into flipped
+    // exercise decisions on near-boundary training paths (a few tenths of an
+    // SE on some trades), so the mixed mode trains exactly as the FP64 mode
+    // and runs only the valuation in FP32.  The network policy keeps its
+    // FP32 store (its own tolerance is the polynomial policy's SE).
+    const bool fp32_store = backend != PE_BACKEND_GPU_FP64 && plans[0]->lsm_trade.basis == kLsmMlp;
+    const size_t real = fp32_store ? sizeof(float) : sizeof(double);
     std::vector<LsmDate> dates;
     std::vector<uint32_t> sobol;
     uint64_t store_elems = 0, cash_elems = 0;
@@ -1451,11 +1458,15 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
                   : launch_lsm_mlp_f32(train, train.n_items, L.mlp_args, L.mlp_wgs, value, value.n_items, L.risk, t_stream,
                                        L.ev[3], L.ev[1]),
               "MLP LSM launch");
-    else
-        check(backend == PE_BACKEND_GPU_FP64
-                  ? launch_lsm_f64(train, train.n_items, value, value.n_items, L.basis, L.risk, t_stream, L.ev[3], L.ev[1])
-                  : launch_lsm_f32(train, train.n_items, value, value.n_items, L.basis, L.risk, t_stream, L.ev[3], L.ev[1]),
+    else if (backend == PE_BACKEND_GPU_FP64)
+        check(launch_lsm_f64(train, train.n_items, value, value.n_items, L.basis, L.risk, t_stream, L.ev[3], L.ev[1]),
               "LSM kernel launch");
+    else {
+        // FP64 training (bit-identical decisions to the FP64 mode), FP32 valuation
+        check(launch_lsm_f64(train, train.n_items, value, 0, L.basis, false, t_stream, L.ev[3], L.ev[1]),
+              "LSM training launch");
+        check(launch_lsm_value_f32(value, value.n_items, L.basis, L.risk, t_stream), "LSM valuation launch");
+    }
     check(hipEventRecord(L.ev[2], t_stream), "hipEventRecord");
     check(hipEventSynchronize(L.ev[2]), "hipEventSynchronize");
     float train_ms = 0.0f, value_ms = 0.0f, path_ms = 0.0f;

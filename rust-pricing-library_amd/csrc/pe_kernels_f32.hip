@@ -68,6 +68,10 @@ hipError_t launch_rough_f32(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 
     return launch_rough<float>(a, pa, grid, payoff_items, error, s);
 }
 
+hipError_t launch_lsm_value_f32(const LsmArgs& value, uint32_t value_items, uint32_t basis, bool risk, hipStream_t s) {
+    return launch_lsm_value<float>(value, value_items, basis, risk, s);
+}
+
 hipError_t launch_lsm_f32(const LsmArgs& train, uint32_t train_items, const LsmArgs& value, uint32_t value_items,
                           uint32_t basis, bool risk, hipStream_t s, hipEvent_t paths_done, hipEvent_t between) {
     return launch_lsm<float>(train, train_items, value, value_items, basis, risk, s, paths_done, between);

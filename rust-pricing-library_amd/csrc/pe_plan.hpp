@@ -288,6 +288,7 @@ hipError_t launch_lv_risk_f64(const LvArgs& a, dim3 grid, size_t lds, int* err, 
 hipError_t launch_lv_risk_f32(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s);
 hipError_t launch_lsm_f64(const LsmArgs& train, uint32_t train_items, const LsmArgs& value, uint32_t value_items,
                           uint32_t basis, bool risk, hipStream_t s, hipEvent_t paths_done, hipEvent_t between);
+hipError_t launch_lsm_value_f32(const LsmArgs& value, uint32_t value_items, uint32_t basis, bool risk, hipStream_t s);
 hipError_t launch_lsm_f32(const LsmArgs& train, uint32_t train_items, const LsmArgs& value, uint32_t value_items,
                           uint32_t basis, bool risk, hipStream_t s, hipEvent_t paths_done, hipEvent_t between);
 hipError_t launch_lsm_mlp_f64(const LsmArgs& train, uint32_t train_items, const MlpLsmArgs& m, uint32_t wgs,

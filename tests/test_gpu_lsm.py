@@ -204,10 +204,18 @@ def test_batch_equals_single_plans_and_oracle():
 
 
 def test_laguerre_batch_mixed_within_tenth_se():
-    reqs = workloads.american_batch(16, seed=9, train_log2=14, value_log2=14, dates=50)
+    """The full 256-trade x 50-date American workload at 2^18 training and
+    2^18 valuation paths per trade: every mixed price within 0.1 SE of the
+    FP64 mode.  The mixed mode trains on FP64 paths (decisions identical to
+    the FP64 mode) and values in FP32."""
+    reqs = workloads.american_batch(256, train_log2=18, value_log2=18, dates=50)
     plans = [engine.PricingPlan.compile(r) for r in reqs]
-    f64, _ = engine.evaluate_lsm(plans, "fp64")
-    mixed, _ = engine.evaluate_lsm(plans, "mixed")
+    f64, frep = engine.evaluate_lsm(plans, "fp64")
+    mixed, mrep = engine.evaluate_lsm(plans, "mixed")
+    assert [d.coefficients[:] for d in frep[7].decisions] == [d.coefficients[:] for d in mrep[7].decisions]
+    worst = max(abs(a.estimates[0].value - b.estimates[0].value) / a.estimates[0].standard_error
+                for a, b in zip(f64, mixed))
+    print("max mixed - fp64 in SE: %.4f" % worst, flush=True)
     for r, a, b in zip(reqs, f64, mixed):
         se = a.estimates[0].standard_error
         assert abs(a.estimates[0].value - b.estimates[0].value) < 0.1 * se, (r.trade_id, a.estimates[0].value,
