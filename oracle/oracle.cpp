@@ -1988,7 +1988,169 @@ int oracle_rough_evaluate(const pe_request* const* reqs, uint32_t count, const p
 
 }  // extern "C"
 
+// ------------------------------------------ early exercise on lifted Heston --
+// No reference counterpart: the reference's hybrid adapters reject early
+// exercise (crates/pricing/src/engine/compile/simulation.rs:55-59).  The
+// reference's own least-squares policy -- train_exercise_policy and
+// value_exercise_policy (crates/pricing/src/engine/mc/lsm.rs:1347-1695) with
+// standardized monomials of total degree <= max_degree and CPQR -- is run on
+// the features (spot, variance) of lifted-Heston paths at the exercise dates
+// (lifted_heston_path, rough_volatility/mod.rs:354-392, 505-547).  Training
+// paths use the training engine in domain LsmTrain, valuation paths the
+// request's engine in domain Valuation; trajectories are ordered (scramble,
+// unit, antithetic pass) as lsm_matrices orders them.
+static LsmMatrices rough_lsm_matrices(const RoughPlan& rp, const std::vector<size_t>& nodes, const pe_engine& e,
+                                      uint32_t domain) {
+    const Plan& b = *rp.bases[0];
+    const pe_product& pr = b.req.product;
+    const size_t dates = nodes.size();
+    const size_t n = rp.grid.steps();
+    const uint32_t dim = static_cast<uint32_t>(2 * n);
+    const size_t mult = e.antithetic ? 2 : 1;
+    LsmMatrices m;
+    m.paths = static_cast<size_t>(e.kind == PE_ENGINE_RQMC ? e.scrambles : 1u) * e.units * mult;
+    m.immediate.assign(dates * m.paths, 0.0);
+    m.features.assign((dates - 1) * m.paths * 2, 0.0);
+    std::vector<double> z(dim), mate(dim), forwards, variance, factors;
+    auto write = [&](const std::vector<double>& zz, size_t path) {
+        lifted_heston_path(rp.model, rp.grid.nodes, b.market.spot, zz.data(), forwards, variance, factors);
+        for (size_t d = 0; d < dates; ++d) {
+            const size_t node = nodes[d];
+            const double spot = rp.scale[node] * forwards[node];
+            const double signed_intrinsic = pr.side == PE_CALL ? spot - pr.strike : pr.strike - spot;
+            m.immediate[d * m.paths + path] = (signed_intrinsic >= 0.0 ? signed_intrinsic : 0.0) * pr.notional;
+            if (d + 1 == dates) continue;
+            m.features[(d * m.paths + path) * 2] = spot;
+            m.features[(d * m.paths + path) * 2 + 1] = variance[node];
+        }
+    };
+    auto both = [&](size_t base) {
+        write(z, base);
+        if (mult == 2) {
+            for (size_t i = 0; i < z.size(); ++i) mate[i] = -z[i];
+            write(mate, base + 1);
+        }
+    };
+    if (e.kind == PE_ENGINE_PSEUDO) {
+        for (uint64_t u = 0; u < e.units; ++u) {
+            for (uint32_t d = 0; d < dim; ++d) z[d] = philox_normal(e.seed, u, d, static_cast<Domain>(domain));
+            both(u * mult);
+        }
+    } else {
+        RqmcPlan q;
+        q.compile(g_directions, e.units, e.scrambles, e.seed, dim);
+        for (uint32_t sc = 0; sc < e.scrambles; ++sc)
+            for (uint64_t u = 0; u < e.units; ++u) {
+                for (uint32_t d = 0; d < dim; ++d) z[d] = inverse_standard_normal(q.uniform(sc, u, d));
+                both((static_cast<size_t>(sc) * e.units + u) * mult);
+            }
+    }
+    return m;
+}
+
+static void evaluate_rough_american(const RoughPlan& rp, pe_result* out, LsmOutcome* keep) {
+    const Plan& b = *rp.bases[0];
+    const pe_lsm& l = b.req.lsm;
+    const pe_engine& ve = rp.engine;
+    const pe_engine& te = l.training_engine;
+    if (te.kind != ve.kind) throw std::invalid_argument("training and valuation engines must be of one kind");
+    if (l.method != PE_LSM_POLY_CPQR) throw std::invalid_argument("lifted Heston early exercise: CPQR policy only");
+    if (ve.brownian_bridge || te.brownian_bridge) throw std::invalid_argument("no Brownian bridge for rough LSM");
+    LsmSettings s;
+    s.basis.init(2, l.max_degree);
+    s.itm_only = true;
+    s.abs_tol = l.abs_rank_tolerance;
+    s.rel_tol = l.rel_rank_tolerance;
+    s.itm_tol = l.itm_abs_tolerance;
+    s.max_elements = l.max_matrix_elements ? l.max_matrix_elements : UINT64_MAX;
+    const std::vector<size_t>& nodes = rp.obs_node[0];
+    const size_t dates = nodes.size();
+    std::vector<double> discount;
+    for (size_t d = 0; d < dates; ++d) discount.push_back(b.market.discount_curve.discount(b.obs_times[d]));
+    LsmOutcome o;
+    {
+        const LsmMatrices train = rough_lsm_matrices(rp, nodes, te, kLsmTrain);
+        o.trained = train_exercise_policy(s, dates, train.paths, train.immediate, train.features, discount);
+    }
+    const LsmMatrices val = rough_lsm_matrices(rp, nodes, ve, kValuation);
+    o.valued = value_exercise_policy(o.trained.decisions, dates, val.paths, 2, val.immediate, val.features, discount);
+    std::vector<double> in_sample(o.trained.realized_cashflows.size());
+    for (size_t i = 0; i < in_sample.size(); ++i)
+        in_sample[i] = o.trained.realized_cashflows[i] * discount[o.trained.stopping_indices[i]];
+    uint64_t independent;
+    if (ve.kind == PE_ENGINE_PSEUDO) {
+        o.unit_values = collapse_antithetic(o.valued.discounted_cashflows, ve.antithetic != 0);
+        independent = ve.units;
+        const std::vector<double> ins = collapse_antithetic(in_sample, te.antithetic != 0);
+        o.in_sample_value = Statistics::two_pass(ins).sum.total() / static_cast<double>(ins.size());
+    } else {
+        o.unit_values = rqmc_replicates(o.valued.discounted_cashflows, ve);
+        independent = ve.scrambles;
+        const std::vector<double> ins = rqmc_replicates(in_sample, te);
+        o.in_sample_value = Statistics::two_pass(ins).sum.total() / static_cast<double>(ins.size());
+    }
+    const Statistics st = Statistics::two_pass(o.unit_values);
+    std::memset(out, 0, sizeof(*out));
+    out->estimates[PE_C_PRICE] = estimate(st, independent, 1.0);
+    out->component_sum[PE_C_PRICE] = st.sum.total();
+    out->component_mean[PE_C_PRICE] = st.moments.mean;
+    out->component_m2[PE_C_PRICE] = st.moments.m2;
+    const double sv = st.moments.has_variance() ? st.moments.sample_variance() : 0.0;
+    out->sampling_variance = sv;
+    out->estimator_variance = sv / static_cast<double>(independent);
+    out->independent_units = independent;
+    out->evaluated_paths = val.paths;
+    out->time_steps = static_cast<uint32_t>(rp.grid.steps());
+    out->path_steps = val.paths * rp.grid.steps();
+    if (keep) *keep = std::move(o);
+}
+
 extern "C" {
+
+// Early exercise on lifted-Heston paths (see evaluate_rough_american): one
+// American carrier; `diag` rows of 40 values per non-terminal date: itm rows,
+// rank, rank threshold, residual sum of squares, (mean, population variance)
+// of spot and of variance, coefficients[6], |R_pp|[6], pivots[6],
+// continue-all, spot inactive, variance inactive; returns the date count.
+int oracle_rough_lsm_evaluate(const pe_request* carrier, const pe_lifted_heston* model, double maximum_step,
+                              const pe_exec_policy* policy, pe_result* out, double* diag, uint32_t capacity,
+                              double* in_sample, uint64_t* exercise_counts, uint32_t* stops, uint64_t stop_capacity) {
+    try {
+        RoughPlan rp;
+        compile_rough(rp, &carrier, 1, *model, maximum_step, policy);
+        if (!rp.bases[0]->american) throw std::invalid_argument("not an early-exercise product");
+        LsmOutcome o;
+        evaluate_rough_american(rp, out, &o);
+        if (in_sample) *in_sample = o.in_sample_value;
+        if (exercise_counts)
+            for (size_t d = 0; d < o.valued.exercise_counts.size(); ++d) exercise_counts[d] = o.valued.exercise_counts[d];
+        for (size_t i = 0; stops && i < o.valued.stopping_indices.size() && i < stop_capacity; ++i)
+            stops[i] = static_cast<uint32_t>(o.valued.stopping_indices[i]);
+        const size_t n = o.trained.decisions.size();
+        for (size_t d = 0; d < n && d < capacity && diag; ++d) {
+            double* r = diag + 40 * d;
+            for (int k = 0; k < 40; ++k) r[k] = 0.0;
+            const Decision& dec = o.trained.decisions[d];
+            r[0] = static_cast<double>(dec.itm_rows);
+            r[26] = dec.continue_all ? 1.0 : 0.0;
+            if (dec.continue_all) continue;
+            const Regression& m = dec.model;
+            r[1] = static_cast<double>(m.rank);
+            r[2] = m.rank_threshold;
+            r[3] = m.residual_sum_squares;
+            r[4] = m.scalings[0].mean;
+            r[5] = m.scalings[0].population_variance;
+            r[6] = m.scalings[1].mean;
+            r[7] = m.scalings[1].population_variance;
+            for (size_t k = 0; k < 6 && k < m.coefficients.size(); ++k) r[8 + k] = m.coefficients[k];
+            for (size_t k = 0; k < 6 && k < m.diagonal_abs.size(); ++k) r[14 + k] = m.diagonal_abs[k];
+            for (size_t k = 0; k < 6 && k < m.pivot_order.size(); ++k) r[20 + k] = static_cast<double>(m.pivot_order[k]);
+            r[27] = m.scalings[0].inactive ? 1.0 : 0.0;
+            r[28] = m.scalings[1].inactive ? 1.0 : 0.0;
+        }
+        return static_cast<int>(n);
+    } catch (const std::exception& e) { return -fail(e, 1); }
+}
 
 // The American plan's exercise policy: per non-terminal date, `row` of
 // `diag` holds (itm rows, rank, rank threshold, residual sum of squares,

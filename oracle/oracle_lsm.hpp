@@ -401,6 +401,7 @@ struct LaguerreModel {
 struct Decision {  // ExerciseDecisionModel
     bool continue_all = false;
     bool laguerre = false;
+    bool itm_only = false;   // Longstaff--Schwartz rule for a monomial policy
     double itm_tol = 0.0;
     Regression model;
     LaguerreModel lag;
@@ -410,7 +411,7 @@ struct Decision {  // ExerciseDecisionModel
     bool should_exercise(double immediate, const double* f) const {
         if (!std::isfinite(immediate)) throw std::runtime_error("non-finite decision value");
         if (continue_all) return false;
-        if (laguerre && !(immediate > itm_tol)) return false;
+        if ((laguerre || itm_only) && !(immediate > itm_tol)) return false;
         return immediate > continuation(f);
     }
 };
@@ -420,6 +421,7 @@ struct LsmSettings {
     double abs_tol = 1e-14, rel_tol = 1e-12, itm_tol = 0.0;
     uint64_t max_elements = UINT64_MAX;
     bool laguerre = false;   // weighted Laguerre + Gram/Cholesky instead of monomials + CPQR
+    bool itm_only = false;   // monomials + CPQR, exercising in-the-money paths only
     double strike = 1.0;
 };
 
@@ -457,6 +459,8 @@ inline Decision fit_exercise_decision(const LsmSettings& s, const double* featur
         return d;
     }
     d.model = fit_polynomial_regression(s.basis, f, itm, t, s.abs_tol, s.rel_tol);
+    d.itm_only = s.itm_only;
+    d.itm_tol = s.itm_tol;
     return d;
 }
 

@@ -248,6 +248,35 @@ def rough_smile(points=1 << 16, scrambles=16, strikes=21, maturities=SMILE_MATUR
 
 
 AMERICAN_RATE = 0.03
+BERMUDAN_RATE = 0.05
+
+
+def rough_bermudan(units=1 << 18, train_units=1 << 18, dates=12, strike=100.0, rate=BERMUDAN_RATE, expiry_days=365,
+                   seed=0xBE5D_0001, train_seed=0xBE5D_0002, method=abi.PE_LSM_POLY_CPQR, max_degree=2,
+                   scrambles=0, antithetic=False, mlp_epochs=4, mlp_batch=8192):
+    """Bermudan put on the lifted-Heston smile model: S0 = 100, K = 100,
+    `dates` exercise dates at round(expiry_days k / dates) days, r = 5%
+    (an American put with r = 0 never exercises early), q = 0; pseudo-MC
+    valuation paths (domain Valuation) and independent training paths
+    (domain LsmTrain), or RQMC with `scrambles`.  The regression features are
+    the spot and the variance; the basis is the standardized monomials of
+    total degree <= max_degree (CPQR policy) or the network policy."""
+    days = sorted({max(1, int(round(expiry_days * (k + 1) / dates))) for k in range(dates)})
+    horizon = max(1.0, days[-1] / 365.0)
+    req = request.Request()
+    req.set_market(100.0, [0.0, horizon], [1.0, math.exp(-rate * horizon)], [0.0, horizon], [1.0, 1.0])
+    req.set_black_scholes(0.2)
+    req.set_product(abi.PE_PRODUCT_AMERICAN, abi.PE_PUT, days[-1], strike, dates=days)
+    if scrambles:
+        req.set_rqmc(seed, units, scrambles, antithetic=antithetic)
+        training = (abi.PE_ENGINE_RQMC, train_seed, train_units, scrambles)
+    else:
+        req.set_pseudo(seed, units, antithetic=antithetic)
+        training = (abi.PE_ENGINE_PSEUDO, train_seed, train_units, 0)
+    req.set_risk()
+    req.set_lsm(training, antithetic=antithetic, method=method, max_degree=max_degree, mlp_epochs=mlp_epochs,
+                mlp_batch=mlp_batch)
+    return req
 
 
 def american_batch(count=256, seed=0x5EED_1A5E, train_log2=18, value_log2=18, dates=50,

@@ -143,6 +143,27 @@ def oracle_lsm(req, block=256, workers=4):
     return res, [list(diag[20 * d:20 * d + 20]) for d in range(n)], ins.value, list(counts)
 
 
+def oracle_rough_lsm(req, model, maximum_step, block=256, workers=4, stop_capacity=0):
+    """The oracle's early exercise on lifted-Heston paths (features spot and
+    variance): (result, per-date decision rows of 40 values, in-sample value,
+    valuation exercise counts, stopping indices or None)."""
+    from pricing_amd import abi, request
+    res = abi.pe_result()
+    diag = (C.c_double * (40 * 1024))()
+    ins = C.c_double()
+    counts = (C.c_uint64 * 1024)()
+    stops = (C.c_uint32 * max(stop_capacity, 1))()
+    pol = request.exec_policy(workers, block)
+    lib = oracle()
+    n = lib.oracle_rough_lsm_evaluate(req.pointer(), C.byref(model), C.c_double(maximum_step), C.byref(pol),
+                                      C.byref(res), diag, 1024, C.byref(ins), counts, stops,
+                                      C.c_uint64(stop_capacity))
+    if n < 0:
+        raise RuntimeError(lib.oracle_last_error().decode())
+    return (res, [list(diag[40 * d:40 * d + 40]) for d in range(n)], ins.value, list(counts),
+            list(stops)[:stop_capacity] if stop_capacity else None)
+
+
 def oracle_lsm_dual(req, continuation_paths, exercise_paths, inner_seed, block=64, workers=4):
     """The oracle's Andersen-Broadie bounds: (lower, upper, gap) estimates."""
     from pricing_amd import abi, request
