@@ -14,8 +14,9 @@ Workload (one "step" = one evaluation of the whole job):
 A second block ("batch_bs") times the batched BS/Black-76 workload: 4096
 uneven trades with affine dividends and Delta/Vega, mixed vs FP64 device modes
 and the CPU oracle on a sample of the trades.  A third ("rough_smile") times
-the lifted Heston 21 x 8 European smile on one 2^20-path RQMC set.  A fourth
-("lsm_american") prices 256 American puts by LSM with the weighted-Laguerre
+the lifted Heston 21 x 8 European smile on one 2^20-path RQMC set, and
+("rough_bermudan") a Bermudan put on the same model by least squares on
+(spot, variance).  A fourth ("lsm_american") prices 256 American puts by LSM with the weighted-Laguerre
 policy and with the learned (MLP) policy.
 
 Prints one JSON line.  --gpus above 1 is refused: this build runs one GPU.
@@ -280,6 +281,75 @@ def rough_block(steps, cpu_seconds, no_cpu, no_fp64):
     return block
 
 
+def rough_bermudan_block(cpu_seconds, no_cpu, no_fp64):
+    """Config 5 early exercise: a Bermudan put (12 dates, one year, K = S0 =
+    100, r = 5%) on the smile model's lifted Heston paths, least squares on
+    (spot, variance) with standardized monomials of total degree 2 and CPQR
+    rules; 2^18 training + 2^18 independent valuation pseudo paths.  The
+    policy trains on FP64 paths in both modes; the mixed mode values on the
+    mixed scheme's paths."""
+    model = workloads.lifted_heston_model(engine.lifted_from_rough(0.1, 20, 2.5))
+    req = workloads.rough_bermudan(units=1 << 18, train_units=1 << 18)
+    plan = engine.RoughPlan.compile(req, model, 2.0 / 256)
+
+    def run(backend, reps):
+        engine.evaluate_lsm([plan], backend)
+        best = None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            res, rep = engine.evaluate_lsm([plan], backend)
+            wall = time.perf_counter() - t0
+            if best is None or wall < best[2]:
+                best = (res[0], rep[0], wall)
+        return best
+
+    mixed, mrep, mwall = run("mixed", 3)
+    steps = mixed.time_steps
+    path_steps = (req.c.lsm.training_engine.units + req.c.engine.units) * steps
+    ms = mrep.training_ms + mrep.valuation_ms
+    block = {
+        "workload": "Bermudan put K=S0=100, r=5%, 12 exercise dates over 1y, lifted Heston H=0.1 (20 factors, "
+                    "ratio 2.5), step 2/256; LSM on (spot, variance), total degree 2, CPQR; 2^18 training + "
+                    "2^18 valuation pseudo paths",
+        "time_steps": steps,
+        "path_steps": path_steps,
+        "device_ms": ms, "training_ms": mrep.training_ms, "path_ms": mrep.path_ms,
+        "valuation_ms": mrep.valuation_ms, "wall_ms": 1e3 * mwall,
+        "value": path_steps / (ms * 1e-3), "unit": "path-steps/s (training + valuation)",
+        "price": mixed.estimates[0].value, "se": mixed.estimates[0].standard_error,
+        "in_sample": mrep.in_sample_value,
+        "early_exercise_share": 1.0 - mrep.exercise_counts[-1] / mrep.valuation_trajectories,
+    }
+    if not no_fp64:
+        f64, frep, fwall = run("fp64", 1)
+        fms = frep.training_ms + frep.valuation_ms
+        block["fp64_device_ms"] = fms
+        block["fp64_valuation_ms"] = frep.valuation_ms
+        block["fp64_price"] = f64.estimates[0].value
+        block["mixed_over_fp64"] = fms / ms
+        block["mixed_minus_fp64_in_se"] = (mixed.estimates[0].value - f64.estimates[0].value) / \
+            f64.estimates[0].standard_error
+    if not no_cpu:
+        from _support import oracle_rough_lsm
+
+        log2 = 10
+        while True:
+            sample = workloads.rough_bermudan(units=1 << log2, train_units=1 << log2)
+            t0 = time.perf_counter()
+            res, _, _, _, _ = oracle_rough_lsm(sample, model, 2.0 / 256)
+            dt = time.perf_counter() - t0
+            if dt >= 0.25 * cpu_seconds or log2 >= 18:
+                break
+            log2 = min(18, log2 + max(1, min(4, int(cpu_seconds / max(dt, 1e-3)).bit_length() - 1)))
+        block["cpu_baseline"] = {
+            "value": 2 * (1 << log2) * res.time_steps / dt, "unit": "path-steps/s (training + valuation)",
+            "cores": 1, "kind": "port",
+            "sample": f"the same Bermudan at 2^{log2} training + 2^{log2} valuation paths, FP64 oracle "
+                      f"(serial path generation and regression), {dt:.2f} s wall"}
+        block["speedup_vs_cpu_one_core"] = block["value"] / block["cpu_baseline"]["value"]
+    return block
+
+
 def lsm_block(cpu_seconds, no_cpu, no_fp64, no_mlp=False):
     """Config 4: 256 American puts by LSM, 50 exercise dates, 2^18 training +
     2^18 independent valuation paths per trade.  Plain policy: weighted
@@ -306,33 +376,34 @@ def lsm_block(cpu_seconds, no_cpu, no_fp64, no_mlp=False):
     value_steps = sum(r.c.engine.units * r.c.product.date_count for r in reqs)
     pms = prep[0].training_ms + prep[0].valuation_ms
     # Rooflines of the two stored-path kernels (HBM-bound by design).  The
-    # path kernel writes the date-major FP32 training store once: 4 B per
-    # (trade, date, path).  The induction kernel's algorithmic traffic: the
-    # store read once (4 B per path-date) plus the FP32 cash flow read and
-    # written once per non-terminal date (8 B per path-date).
+    # policy trains on FP64 paths in both modes.  The path kernel writes the
+    # date-major FP64 training store once: 8 B per (trade, date, path).  The
+    # induction kernel's algorithmic traffic: the store read once (8 B per
+    # path-date) plus the FP64 cash flow read and written once per
+    # non-terminal date (16 B per path-date).
     rep = prep[0]
     paths = sum(r.c.lsm.training_engine.units for r in reqs)
     dates = reqs[0].c.product.date_count
     induction_ms = rep.training_ms - rep.path_ms
-    induction_bytes = paths * (4 * dates + 8 * (dates - 1))
-    store_bytes = paths * dates * 4
+    induction_bytes = paths * (8 * dates + 16 * (dates - 1))
+    store_bytes = paths * dates * 8
     traffic = lsm_traffic()
     block = {
         "workload": "256 American puts, S0=100, K~U[80,120], sigma~U[0.2,0.4], r=3%, T=1y, 50 exercise dates, "
                     "2^18 training + 2^18 valuation pseudo paths per trade",
         "roofline": {
-            "bound": "hbm", "kernel": "lsm_induction_kernel<float, 1u>", "avg_kernel_ms": induction_ms,
+            "bound": "hbm", "kernel": "lsm_induction_kernel<double, 1u>", "avg_kernel_ms": induction_ms,
             "algorithmic_bytes": induction_bytes,
             "achieved": induction_bytes / (induction_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": induction_bytes / (induction_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "traffic": traffic.get("lsm_induction_kernel<float, 1u>"),
+            "traffic": traffic.get("lsm_induction_kernel<double, 1u>"),
         },
         "path_store_roofline": {
-            "bound": "hbm", "kernel": "lsm_train_paths_kernel<float>", "avg_kernel_ms": rep.path_ms,
+            "bound": "hbm", "kernel": "lsm_train_paths_kernel<double>", "avg_kernel_ms": rep.path_ms,
             "algorithmic_bytes": store_bytes,
             "achieved": store_bytes / (rep.path_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": store_bytes / (rep.path_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "traffic": traffic.get("lsm_train_paths_kernel<float>"),
+            "traffic": traffic.get("lsm_train_paths_kernel<double>"),
         },
         "poly_laguerre": {
             "device_ms": pms, "training_ms": prep[0].training_ms, "valuation_ms": prep[0].valuation_ms,
@@ -507,6 +578,7 @@ def main():
         out["batch_bs"] = batch_block(3, args.cpu_seconds, args.no_cpu, args.no_fp64)
     if not args.no_rough:
         out["rough_smile"] = rough_block(5, args.cpu_seconds, args.no_cpu, args.no_fp64)
+        out["rough_bermudan"] = rough_bermudan_block(args.cpu_seconds, args.no_cpu, args.no_fp64)
     if not args.no_graph:
         out["graph_payoff"] = graph_block(5, args.no_fp64)
     if not args.no_lsm:

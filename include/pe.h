@@ -448,14 +448,14 @@ typedef struct pe_lsm_decision {
     int32_t continue_all;           /* no in-the-money training row */
     uint32_t rank;
     uint64_t itm_rows;
-    double coefficients[4];
-    double diagonal_abs[4];
-    uint32_t pivot_order[4];
+    double coefficients[6];         /* per basis column (spot policies: 4; spot and variance: 1 s v s^2 sv v^2) */
+    double diagonal_abs[6];
+    uint32_t pivot_order[6];        /* original column per pivot position (active columns) */
     double feature_mean;            /* FeatureScaling of the spot (monomial basis) */
     double feature_population_variance;
     double feature_scale;
     int32_t feature_inactive;
-    int32_t pad;
+    uint32_t active_columns;        /* pivot entries in use */
     double rank_threshold;
     double residual_sum_squares;
     /* MLP policy: the exercise region in moneyness S/K at this date, for
@@ -465,6 +465,13 @@ typedef struct pe_lsm_decision {
     uint32_t region_cut_count;
     float region_cuts[64];
     double zero_scale_threshold;    /* FeatureScaling: 64 eps max(1, max |ITM spot|) (monomial basis) */
+    /* lifted-Heston policies: FeatureScaling of the variance state */
+    double variance_mean;
+    double variance_population_variance;
+    double variance_scale;
+    double variance_zero_scale_threshold;
+    int32_t variance_inactive;
+    uint32_t feature_count;         /* 1: spot; 2: spot and variance */
 } pe_lsm_decision;
 
 /* Early-exercise diagnostics of one plan (EarlyExerciseDiagnostics,
@@ -583,6 +590,14 @@ pe_status pe_lifted_heston_from_rough(double hurst, uint32_t factors, double rat
  * counterpart). */
 pe_status pe_rough_plan_compile(const pe_request* carrier, const pe_lifted_heston* model, double maximum_step,
                                 const pe_exec_policy* policy, pe_plan** out);
+/* An American carrier (PE_PRODUCT_AMERICAN: a Bermudan on its exercise
+ * dates) under lifted Heston is priced by least squares with the features
+ * (spot, variance state): the reference's regression (standardized
+ * monomials of total degree lsm.max_degree <= 2, CPQR rules) on in-the-money
+ * rows, exercising in-the-money paths only; training paths in FP64 in both
+ * modes.  Evaluate with pe_plan_evaluate or pe_lsm_evaluate (one plan per
+ * launch).  No reference counterpart: the reference rejects early exercise
+ * in its hybrid adapters (crates/pricing/src/engine/compile/simulation.rs:55-59). */
 
 /* LocalVarianceGrid::from_surface for a StandardSsvi surface with a ThetaPchip
  * total-variance curve and a power-law phi, computed in FP64 on the device

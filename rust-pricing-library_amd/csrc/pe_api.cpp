@@ -279,9 +279,6 @@ void BsLaunchDeleter::operator()(BsLaunch* l) const { delete l; }
 
 namespace {
 
-// Launch cache per plan set, owned by the set's first plan.
-std::mutex g_bs_cache_mutex;
-
 BsLaunch& prepare_bs(std::vector<HostPlan*>& plans, pe_backend backend) {
     HostPlan& p0 = *plans[0];
     std::vector<uint64_t> key;
@@ -640,6 +637,26 @@ struct RoughLaunch {
 
 void RoughLaunchDeleter::operator()(RoughLaunch* l) const { delete l; }
 
+// Device state of a Bermudan under lifted Heston (one plan per launch).
+struct RoughLsmLaunch {
+    int backend = 0;
+    RoughLsmArgs args{};
+    dim3 train_grid, value_grid;
+    uint32_t n = 0, chunks = 0, scrambles = 1, dates = 0;
+    DeviceBuffer d_steps, d_den, d_rden, d_train_sobol, d_value_sobol, d_nodes, d_discount, d_scale, store_s, store_v,
+        cash, partials, scalings, decisions, value_partials, counts, stops, in_sample, err;
+    std::vector<BlockStat> host_partials;
+    std::vector<RoughLsmDecision> host_decisions;
+    std::vector<unsigned long long> host_counts;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};   // start, paths done, training done, valuation done
+    ~RoughLsmLaunch() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+void RoughLsmLaunchDeleter::operator()(RoughLsmLaunch* l) const { delete l; }
+
 namespace {
 
 void prepare_rough(std::vector<HostPlan*>& plans, pe_backend backend);
@@ -683,6 +700,54 @@ void evaluate_rough(std::vector<HostPlan*>& plans, pe_backend backend, pe_result
     g_kernel_launches = 2;
 }
 
+// Physical spot per unit of F at time t (HullWhiteDividendPlan node scale
+// without dividends, crates/pricing/src/models/hull_white_dividends.rs:228-235).
+double rough_scale(const HostPlan& p, double t) {
+    const double pd = std::exp(p.discount_curve.log_discount(t));
+    const double qd = std::exp(p.dividend_curve.log_discount(t));
+    return 1.0 * p.spot / p.spot * qd / pd;
+}
+
+// The grid and model fields of RoughArgs: per-step dt, sqrt(dt) and the
+// right node's spot scale, the [step][factor] tables 1 + x_i dt and their
+// reciprocals, the lifted-Heston parameters.
+void rough_grid_args(const HostPlan& p, const std::vector<double>& nodes, DeviceBuffer& d_steps, DeviceBuffer& d_den,
+                     DeviceBuffer& d_rden, RoughArgs& a) {
+    const HostLifted& h = p.lifted;
+    const uint32_t n = static_cast<uint32_t>(nodes.size() - 1);
+    const uint32_t nf = static_cast<uint32_t>(h.weights.size());
+    std::vector<RoughStep> steps(n);
+    std::vector<double> den(static_cast<size_t>(n) * nf);
+    std::vector<double> rden(den.size());
+    for (uint32_t j = 0; j < n; ++j) {
+        const double dt = nodes[j + 1] - nodes[j];
+        steps[j] = RoughStep{dt, std::sqrt(dt), rough_scale(p, nodes[j + 1]), 0.0};
+        for (uint32_t i = 0; i < nf; ++i) {
+            const double d = 1.0 + h.rates[i] * dt;
+            if (!std::isfinite(d)) throw PlanError(PE_ERR_INVALID_ARGUMENT, "lifted time scale overflow");
+            den[static_cast<size_t>(j) * nf + i] = d;
+            rden[static_cast<size_t>(j) * nf + i] = 1.0 / d;
+        }
+    }
+    d_steps.upload(steps);
+    d_den.upload(den);
+    d_rden.upload(rden);
+    a.steps = d_steps.as<RoughStep>();
+    a.denominators = d_den.as<double>();
+    a.reciprocals = d_rden.as<double>();
+    a.spot = p.spot;
+    a.scale0 = rough_scale(p, 0.0);
+    a.v0 = h.v0;
+    a.kappa = h.kappa;
+    a.theta = h.theta;
+    a.nu = h.nu;
+    a.rho = h.rho;
+    a.rho_bar = std::sqrt(std::max(1.0 - h.rho * h.rho, 0.0));
+    for (uint32_t i = 0; i < nf; ++i) a.weights[i] = h.weights[i];
+    a.n_steps = n;
+    a.n_factors = nf;
+}
+
 void prepare_rough(std::vector<HostPlan*>& plans, pe_backend backend) {
     HostPlan& p0 = *plans[0];
     const pe_engine& e = p0.req.engine;
@@ -703,27 +768,6 @@ void prepare_rough(std::vector<HostPlan*>& plans, pe_backend backend) {
     const std::vector<double> nodes = compile_time_grid(events, p0.rough_max_step);
     const uint32_t n = static_cast<uint32_t>(nodes.size() - 1);
     if (n > 65536) throw PlanError(PE_ERR_SIZE_LIMIT, "rough time grid resource limit");
-    const HostLifted& h = p0.lifted;
-    const uint32_t nf = static_cast<uint32_t>(h.weights.size());
-    // physical spot per unit of F (HullWhiteDividendPlan node scale without dividends)
-    auto scale_at = [&](double t) {
-        const double pd = std::exp(p0.discount_curve.log_discount(t));
-        const double qd = std::exp(p0.dividend_curve.log_discount(t));
-        return 1.0 * p0.spot / p0.spot * qd / pd;
-    };
-    std::vector<RoughStep> steps(n);
-    std::vector<double> den(static_cast<size_t>(n) * nf);
-    std::vector<double> rden(den.size());
-    for (uint32_t j = 0; j < n; ++j) {
-        const double dt = nodes[j + 1] - nodes[j];
-        steps[j] = RoughStep{dt, std::sqrt(dt), scale_at(nodes[j + 1]), 0.0};
-        for (uint32_t i = 0; i < nf; ++i) {
-            const double d = 1.0 + h.rates[i] * dt;
-            if (!std::isfinite(d)) throw PlanError(PE_ERR_INVALID_ARGUMENT, "lifted time scale overflow");
-            den[static_cast<size_t>(j) * nf + i] = d;
-            rden[static_cast<size_t>(j) * nf + i] = 1.0 / d;
-        }
-    }
     // store slots: the distinct expiry nodes of the plans
     std::vector<uint32_t> store_nodes;
     std::vector<uint32_t> plan_node(plans.size());
@@ -747,33 +791,17 @@ void prepare_rough(std::vector<HostPlan*>& plans, pe_backend backend) {
     DeviceBuffer &d_steps = L->d_steps, &d_den = L->d_den, &d_rden = L->d_rden, &d_store_nodes = L->d_store_nodes,
                  &d_store = L->d_store, &d_sobol = L->d_sobol, &d_err = L->d_err, &d_products = L->d_products,
                  &d_items = L->d_items, &d_partials = L->d_partials;
-    d_steps.upload(steps);
-    d_den.upload(den);
-    d_rden.upload(rden);
+    RoughArgs a{};
+    rough_grid_args(p0, nodes, d_steps, d_den, d_rden, a);
     d_store_nodes.upload(store_nodes);
     d_store.reserve(total_paths * store_nodes.size() * (fp64 ? sizeof(double) : sizeof(float)));
     if (rqmc) d_sobol.upload(build_sobol_table(e.seed, e.scrambles, 2u * n));
-    RoughArgs a{};
-    a.steps = d_steps.as<RoughStep>();
-    a.denominators = d_den.as<double>();
-    a.reciprocals = d_rden.as<double>();
     a.sobol = d_sobol.as<uint32_t>();
     a.store_nodes = d_store_nodes.as<uint32_t>();
     a.store = d_store.ptr;
     a.seed = e.seed;
     a.units = e.units;
     a.total_paths = total_paths;
-    a.spot = p0.spot;
-    a.scale0 = scale_at(0.0);
-    a.v0 = h.v0;
-    a.kappa = h.kappa;
-    a.theta = h.theta;
-    a.nu = h.nu;
-    a.rho = h.rho;
-    a.rho_bar = std::sqrt(std::max(1.0 - h.rho * h.rho, 0.0));
-    for (uint32_t i = 0; i < nf; ++i) a.weights[i] = h.weights[i];
-    a.n_steps = n;
-    a.n_factors = nf;
     a.n_store = static_cast<uint32_t>(store_nodes.size());
     a.rqmc = rqmc ? 1u : 0u;
     a.antithetic = e.antithetic ? 1u : 0u;
@@ -828,6 +856,222 @@ void prepare_rough(std::vector<HostPlan*>& plans, pe_backend backend) {
     L->host.resize(items.size());
     (void)fp64;
     p0.rough_launch.reset(L.release());
+}
+
+// A Bermudan under lifted Heston: the training paths, the backward
+// induction over the exercise dates and the valuation (pe_rough_lsm.inc).
+void prepare_rough_lsm(HostPlan& p, pe_backend backend) {
+    if (p.rough_lsm_launch && p.rough_lsm_launch->backend == static_cast<int>(backend)) return;
+    std::unique_ptr<RoughLsmLaunch, RoughLsmLaunchDeleter> L(new RoughLsmLaunch());
+    L->backend = static_cast<int>(backend);
+    const pe_engine& ve = p.req.engine;
+    const pe_engine& te = p.req.lsm.training_engine;
+    const std::vector<double> nodes = compile_time_grid(p.rough_events, p.rough_max_step);
+    const uint32_t n = static_cast<uint32_t>(nodes.size() - 1);
+    const uint32_t dates = static_cast<uint32_t>(p.obs_times.size());
+    RoughLsmArgs& a = L->args;
+    RoughArgs g{};
+    rough_grid_args(p, nodes, L->d_steps, L->d_den, L->d_rden, g);
+    std::vector<uint32_t> ex_nodes(dates);
+    std::vector<double> ex_df(dates), ex_scale(dates);
+    for (uint32_t k = 0; k < dates; ++k) {
+        const auto it = std::lower_bound(nodes.begin(), nodes.end(), p.obs_times[k]);
+        if (it == nodes.end() || *it != p.obs_times[k] || it == nodes.begin())
+            throw PlanError(PE_ERR_INVALID_ARGUMENT, "exercise dates must be future grid nodes");
+        ex_nodes[k] = static_cast<uint32_t>(it - nodes.begin());
+        ex_df[k] = std::exp(p.discount_curve.log_discount(p.obs_times[k]));
+        ex_scale[k] = rough_scale(p, p.obs_times[k]);
+    }
+    L->d_nodes.upload(ex_nodes);
+    L->d_discount.upload(ex_df);
+    L->d_scale.upload(ex_scale);
+    auto engine = [&](const pe_engine& e, uint32_t domain, DeviceBuffer& sobol) {
+        RoughArgs r = g;
+        r.seed = e.seed;
+        r.units = e.units;
+        r.rqmc = e.kind == PE_ENGINE_RQMC ? 1u : 0u;
+        r.antithetic = e.antithetic ? 1u : 0u;
+        r.domain = domain;
+        if (r.rqmc) {
+            sobol.upload(build_sobol_table(e.seed, e.scrambles, 2u * n));
+            r.sobol = sobol.as<uint32_t>();
+        }
+        return r;
+    };
+    a.train = engine(te, 1u, L->d_train_sobol);   // RandomDomain::LsmTrain
+    a.value = engine(ve, 0u, L->d_value_sobol);   // RandomDomain::Valuation
+    const uint32_t train_scr = te.kind == PE_ENGINE_RQMC ? te.scrambles : 1u;
+    const uint32_t value_scr = ve.kind == PE_ENGINE_RQMC ? ve.scrambles : 1u;
+    a.train_paths = te.units * train_scr * (te.antithetic ? 2u : 1u);
+    a.ex_nodes = L->d_nodes.as<uint32_t>();
+    a.ex_discount = L->d_discount.as<double>();
+    a.ex_scale = L->d_scale.as<double>();
+    L->store_s.reserve(static_cast<size_t>(dates) * a.train_paths * sizeof(double));
+    L->store_v.reserve(static_cast<size_t>(dates) * a.train_paths * sizeof(double));
+    L->cash.reserve(a.train_paths * sizeof(double));
+    a.store_s = L->store_s.as<double>();
+    a.store_v = L->store_v.as<double>();
+    a.cash = L->cash.as<double>();
+    // training passes: about 1024 paths per workgroup, at most 4096 workgroups
+    a.n_wg = static_cast<uint32_t>(std::min<uint64_t>(4096, std::max<uint64_t>(1, (a.train_paths + 1023) / 1024)));
+    L->partials.reserve(static_cast<size_t>(a.n_wg) * kRlGram * sizeof(double));
+    a.partials = L->partials.as<double>();
+    L->scalings.reserve(std::max<size_t>(1, dates) * sizeof(RoughLsmScaling));
+    L->decisions.reserve(std::max<size_t>(1, dates) * sizeof(RoughLsmDecision));
+    a.scalings = L->scalings.as<RoughLsmScaling>();
+    a.decisions = L->decisions.as<RoughLsmDecision>();
+    // valuation: one workgroup per (chunk of units, scramble), a multiple of
+    // 256 units per chunk and about 2048 workgroups in all
+    const uint64_t value_units = ve.units * value_scr;
+    a.value_chunk = 256u * std::max<uint64_t>(1, (value_units + 256u * 2048u - 1) / (256u * 2048u));
+    const uint32_t chunks = static_cast<uint32_t>((ve.units + a.value_chunk - 1) / a.value_chunk);
+    L->value_partials.reserve(static_cast<size_t>(chunks) * value_scr * sizeof(BlockStat));
+    a.value_partials = L->value_partials.ptr;
+    L->counts.reserve(dates * sizeof(unsigned long long));
+    a.counts = L->counts.as<unsigned long long>();
+    L->in_sample.reserve(sizeof(double));
+    a.in_sample = L->in_sample.as<double>();
+    L->err.reserve(sizeof(unsigned long long));
+    a.error = L->err.as<unsigned long long>();
+    a.strike = p.req.product.strike;
+    a.notional = p.req.product.notional;
+    a.itm_tol = p.req.lsm.itm_abs_tolerance;
+    a.abs_tol = p.req.lsm.abs_rank_tolerance;
+    a.rel_tol = p.req.lsm.rel_rank_tolerance;
+    a.n_dates = dates;
+    a.call = p.req.product.side == PE_CALL ? 1u : 0u;
+    a.columns = p.req.lsm.max_degree == 1 ? 3u : 6u;
+    a.value_items = chunks * value_scr;
+    const uint64_t train_blocks = (te.units + kBlockThreads - 1) / kBlockThreads;
+    L->train_grid = dim3(static_cast<uint32_t>(std::min<uint64_t>(train_blocks, 1u << 14)), train_scr, 1u);
+    L->value_grid = dim3(chunks, value_scr, 1u);
+    L->n = n;
+    L->chunks = chunks;
+    L->scrambles = value_scr;
+    L->dates = dates;
+    L->host_partials.resize(static_cast<size_t>(chunks) * value_scr);
+    L->host_decisions.resize(dates);
+    L->host_counts.resize(dates);
+    for (hipEvent_t& e : L->ev) check(hipEventCreate(&e), "hipEventCreate");
+    p.rough_lsm_launch = std::move(L);
+}
+
+void evaluate_rough_lsm(HostPlan& p, pe_backend backend, pe_result* out, pe_lsm_report* report) {
+    prepare_rough_lsm(p, backend);
+    RoughLsmLaunch& L = *p.rough_lsm_launch;
+    RoughLsmArgs a = L.args;
+    const uint64_t vtraj = a.value.units * L.scrambles * (a.value.antithetic ? 2u : 1u);
+    const bool want_stops = report && report->stopping_indices && report->stopping_capacity;
+    if (want_stops) {
+        L.stops.reserve(std::max<uint64_t>(1, vtraj) * sizeof(uint32_t));
+        a.stops = L.stops.as<uint32_t>();
+    }
+    const unsigned long long none = ~0ull;
+    check(hipMemcpyAsync(L.err.ptr, &none, sizeof(none), hipMemcpyHostToDevice, t_stream), "error word");
+    const std::vector<unsigned long long> zeros(L.dates, 0ull);
+    check(hipMemcpyAsync(L.counts.ptr, zeros.data(), L.dates * sizeof(unsigned long long), hipMemcpyHostToDevice,
+                         t_stream),
+          "counts");
+    check(hipEventRecord(L.ev[0], t_stream), "hipEventRecord");
+    check(backend == PE_BACKEND_GPU_FP64 ? launch_rough_lsm_f64(a, L.train_grid, L.value_grid, t_stream, L.ev[1], L.ev[2])
+                                         : launch_rough_lsm_f32(a, L.train_grid, L.value_grid, t_stream, L.ev[1], L.ev[2]),
+          "rough LSM launch");
+    check(hipEventRecord(L.ev[3], t_stream), "hipEventRecord");
+    check(hipEventSynchronize(L.ev[3]), "hipEventSynchronize");
+    float path_ms = 0.0f, train_ms = 0.0f, value_ms = 0.0f;
+    check(hipEventElapsedTime(&path_ms, L.ev[0], L.ev[1]), "hipEventElapsedTime");
+    check(hipEventElapsedTime(&train_ms, L.ev[0], L.ev[2]), "hipEventElapsedTime");
+    check(hipEventElapsedTime(&value_ms, L.ev[2], L.ev[3]), "hipEventElapsedTime");
+    unsigned long long err = none;
+    check(copy_on_stream(&err, L.err.ptr, sizeof(err), hipMemcpyDeviceToHost), "error copy");
+    if (err != none) {
+        out->failing_unit = err;
+        throw PlanError(PE_ERR_NONFINITE_STATE, "sampling unit " + std::to_string(err) + ": non-finite lifted Heston path");
+    }
+    check(copy_on_stream(L.host_partials.data(), L.value_partials.ptr, L.host_partials.size() * sizeof(BlockStat),
+                         hipMemcpyDeviceToHost),
+          "partials copy");
+    check(copy_on_stream(L.host_counts.data(), L.counts.ptr, L.dates * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost),
+          "counts copy");
+    if (L.dates > 1)
+        check(copy_on_stream(L.host_decisions.data(), L.decisions.ptr, (L.dates - 1) * sizeof(RoughLsmDecision),
+                             hipMemcpyDeviceToHost),
+              "decisions copy");
+    double in_sample = 0.0;
+    check(copy_on_stream(&in_sample, L.in_sample.ptr, sizeof(double), hipMemcpyDeviceToHost), "in-sample copy");
+    std::memset(out, 0, sizeof(*out));
+    assemble(p, L.host_partials, L.chunks, L.scrambles, 1, 0, 1, out);
+    out->time_steps = L.n;
+    out->path_steps = out->evaluated_paths * L.n;
+    out->sobol_dimension = a.value.rqmc ? 2u * L.n : 0u;
+    out->device_ms = train_ms + value_ms;
+    g_kernel_ms = train_ms + value_ms;
+    g_kernel_launches = 3 + 6 * (L.dates > 1 ? L.dates - 1 : 0);
+    if (!report) return;
+    pe_lsm_report& r = *report;
+    const pe_engine& te = p.req.lsm.training_engine;
+    r.in_sample_value = in_sample;
+    r.training_ms = train_ms;
+    r.valuation_ms = value_ms;
+    r.path_ms = path_ms;
+    r.exercise_dates = L.dates;
+    r.training_random_domain = te.kind == PE_ENGINE_RQMC ? PE_DOMAIN_RQMC_SCRAMBLE : PE_DOMAIN_LSM_TRAIN;
+    r.valuation_random_domain = a.value.rqmc ? PE_DOMAIN_RQMC_SCRAMBLE : PE_DOMAIN_VALUATION;
+    r.training_sampling_units = te.kind == PE_ENGINE_RQMC ? te.scrambles : te.units;
+    r.training_trajectories = a.train_paths;
+    r.valuation_sampling_units = a.value.rqmc ? L.scrambles : a.value.units;
+    r.valuation_trajectories = vtraj;
+    r.training_has_checksums = p.train_diag.has_checksums;
+    std::memcpy(r.training_direction_checksum, p.train_diag.direction_checksum, 32);
+    std::memcpy(r.training_scramble_checksum, p.train_diag.scramble_checksum, 32);
+    r.valuation_has_checksums = p.diag.has_checksums;
+    std::memcpy(r.valuation_direction_checksum, p.diag.direction_checksum, 32);
+    std::memcpy(r.valuation_scramble_checksum, p.diag.scramble_checksum, 32);
+    r.policy_feature_count = 2;
+    r.policy_max_degree = p.req.lsm.max_degree;
+    r.itm_abs_tolerance = p.req.lsm.itm_abs_tolerance;
+    r.abs_rank_tolerance = p.req.lsm.abs_rank_tolerance;
+    r.rel_rank_tolerance = p.req.lsm.rel_rank_tolerance;
+    r.max_matrix_elements = p.req.lsm.max_matrix_elements;
+    if (want_stops) {
+        const uint64_t k = std::min<uint64_t>(r.stopping_capacity, vtraj);
+        check(copy_on_stream(r.stopping_indices, L.stops.ptr, k * sizeof(uint32_t), hipMemcpyDeviceToHost),
+              "stopping indices copy");
+    }
+    for (uint32_t d = 0; d < L.dates && d < r.capacity; ++d) {
+        if (r.exercise_counts) r.exercise_counts[d] = L.host_counts[d];
+        if (r.exercise_probabilities)
+            r.exercise_probabilities[d] = static_cast<double>(L.host_counts[d]) / static_cast<double>(vtraj);
+        if (r.dividend_collisions) r.dividend_collisions[d] = 0;
+        if (!r.decisions || d + 1 >= L.dates) continue;
+        const RoughLsmDecision& s = L.host_decisions[d];
+        pe_lsm_decision& o = r.decisions[d];
+        std::memset(&o, 0, sizeof(o));
+        o.feature_count = 2;
+        o.continue_all = s.continue_all ? 1 : 0;
+        o.itm_rows = s.itm_rows;
+        if (s.continue_all) continue;
+        o.rank = s.rank;
+        o.active_columns = s.active;
+        for (int k = 0; k < kRlCols; ++k) {
+            o.coefficients[k] = s.coef[k];
+            o.diagonal_abs[k] = s.diag[k];
+            o.pivot_order[k] = s.pivot[k];
+        }
+        o.feature_mean = s.mean[0];
+        o.feature_population_variance = s.variance[0];
+        o.feature_scale = s.scale[0];
+        o.feature_inactive = (s.inactive & 1u) ? 1 : 0;
+        o.zero_scale_threshold = s.zero_threshold[0];
+        o.variance_mean = s.mean[1];
+        o.variance_population_variance = s.variance[1];
+        o.variance_scale = s.scale[1];
+        o.variance_inactive = (s.inactive & 2u) ? 1 : 0;
+        o.variance_zero_scale_threshold = s.zero_threshold[1];
+        o.rank_threshold = s.rank_threshold;
+        o.residual_sum_squares = s.rss;
+    }
 }
 
 void upload_lv_tables(HostPlan& p) {
@@ -1584,6 +1828,8 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
             o.itm_rows = s.itm_rows;
             if (s.continue_all) continue;
             o.rank = s.rank;
+            o.feature_count = 1;
+            o.active_columns = (L.basis == kLsmMonomial && s.inactive) ? 1u : t.columns;
             for (int k = 0; k < 4; ++k) {
                 o.coefficients[k] = s.coef[k];
                 o.diagonal_abs[k] = s.diag[k];
@@ -1671,7 +1917,9 @@ pe_status pe_plan_evaluate(const pe_plan* plan, pe_backend backend, pe_result* o
         HostPlan& p = const_cast<pe_plan*>(plan)->host;
         CallStream cs;
         PlanLocks locks({&p});
-        if (p.rough) {
+        if (p.rough && p.american) {
+            evaluate_rough_lsm(p, backend, out, nullptr);
+        } else if (p.rough) {
             std::vector<HostPlan*> plans{&p};
             evaluate_rough(plans, backend, out);
         } else if (p.american) {
@@ -1711,6 +1959,9 @@ pe_status pe_plans_evaluate_shared(const pe_plan* const* plans, uint32_t count, 
         }
         CallStream cs;
         PlanLocks locks(hp);
+        for (const HostPlan* q : hp)
+            if (q->rough && q->american)
+                throw PlanError(PE_ERR_INVALID_ARGUMENT, "a lifted-Heston Bermudan is evaluated on its own paths");
         if (hp[0]->rough) evaluate_rough(hp, backend, out);
         else evaluate_lv(hp, backend, out);
     });
@@ -1753,7 +2004,9 @@ pe_status pe_batch_evaluate(const pe_plan* const* plans, uint32_t count, pe_back
         std::vector<HostPlan*> groups[3], rest;
         std::vector<size_t> where[3], rest_at;
         for (size_t i = 0; i < hp.size(); ++i) {
-            if (hp[i]->american) {
+            if (hp[i]->rough && hp[i]->american) {
+                evaluate_rough_lsm(*hp[i], backend, results + i, nullptr);
+            } else if (hp[i]->american) {
                 const int g = static_cast<int>(hp[i]->req.lsm.method);
                 groups[g].push_back(hp[i]);
                 where[g].push_back(i);
@@ -1789,6 +2042,17 @@ pe_status pe_lsm_evaluate(const pe_plan* const* plans, uint32_t count, pe_backen
         }
         CallStream cs;
         PlanLocks locks(hp);
+        if (hp[0]->rough) {
+            // lifted-Heston Bermudans: one launch per plan
+            for (size_t i = 0; i < hp.size(); ++i) {
+                if (!hp[i]->rough || !hp[i]->american)
+                    throw PlanError(PE_ERR_INVALID_ARGUMENT, "mixing lifted-Heston and Black-Scholes American plans");
+                evaluate_rough_lsm(*hp[i], backend, results + i, reports ? reports + i : nullptr);
+            }
+            return;
+        }
+        for (const HostPlan* q : hp)
+            if (q->rough) throw PlanError(PE_ERR_INVALID_ARGUMENT, "mixing lifted-Heston and Black-Scholes American plans");
         evaluate_lsm(hp, backend, results, reports);
     });
 }

@@ -8,6 +8,7 @@
 #include "pe_bs.inc"
 #include "pe_lv_risk.inc"
 #include "pe_rough.inc"
+#include "pe_rough_lsm.inc"
 #include "pe_lsm.inc"
 #include "pe_mlp.inc"
 #include "pe_mlp_lsm.inc"
@@ -61,6 +62,11 @@ hipError_t add_bs_batch_node_f32(hipGraph_t graph, const BsBatchArgs& a, bool ri
 
 hipError_t launch_bs_batch_f32(const BsBatchArgs& a, bool risk, bool cont, uint32_t graph_regs, hipStream_t s) {
     return launch_bs_batch<float>(a, risk, cont, graph_regs, s);
+}
+
+hipError_t launch_rough_lsm_f32(const RoughLsmArgs& a, dim3 train_grid, dim3 value_grid, hipStream_t s,
+                               hipEvent_t paths_done, hipEvent_t between) {
+    return launch_rough_lsm<float>(a, train_grid, value_grid, s, paths_done, between);
 }
 
 hipError_t launch_rough_f32(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 grid, uint32_t payoff_items,

@@ -4,6 +4,7 @@
 #include "pe_bs.inc"
 #include "pe_lv_risk.inc"
 #include "pe_rough.inc"
+#include "pe_rough_lsm.inc"
 #include "pe_lsm.inc"
 #include "pe_lsm_dual.inc"
 #include "pe_mlp.inc"
@@ -18,6 +19,11 @@ hipError_t add_bs_batch_node_f64(hipGraph_t graph, const BsBatchArgs& a, bool ri
 
 hipError_t launch_bs_batch_f64(const BsBatchArgs& a, bool risk, bool cont, uint32_t graph_regs, hipStream_t s) {
     return launch_bs_batch<double>(a, risk, cont, graph_regs, s);
+}
+
+hipError_t launch_rough_lsm_f64(const RoughLsmArgs& a, dim3 train_grid, dim3 value_grid, hipStream_t s,
+                               hipEvent_t paths_done, hipEvent_t between) {
+    return launch_rough_lsm<double>(a, train_grid, value_grid, s, paths_done, between);
 }
 
 hipError_t launch_rough_f64(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 grid, uint32_t payoff_items,

@@ -533,8 +533,16 @@ void compile_rough_plan(HostPlan& p, const pe_request& carrier, const pe_lifted_
     if (carrier.market.dividend_count) unsupported("discrete dividends under lifted Heston");
     if (carrier.product.kind == PE_PRODUCT_BARRIER && carrier.product.continuous)
         unsupported("continuous barriers in the rough adapter");
-    if (carrier.product.kind != PE_PRODUCT_EUROPEAN && carrier.product.kind != PE_PRODUCT_DIGITAL)
-        unsupported("lifted Heston on the device prices terminal payoffs (European, Digital)");
+    const bool american = carrier.product.kind == PE_PRODUCT_AMERICAN;
+    if (carrier.product.kind != PE_PRODUCT_EUROPEAN && carrier.product.kind != PE_PRODUCT_DIGITAL && !american)
+        unsupported("lifted Heston on the device prices European, Digital and Bermudan (American) payoffs");
+    if (american) {
+        const pe_lsm& l = carrier.lsm;
+        if (l.method != PE_LSM_POLY_CPQR) unsupported("lifted Heston early exercise: CPQR monomial policy only");
+        if (l.max_degree < 1 || l.max_degree > 2) unsupported("lifted Heston early exercise: total degree 1 or 2");
+        if (carrier.engine.brownian_bridge || l.training_engine.brownian_bridge)
+            invalid("no Brownian bridge for lifted Heston early exercise");
+    }
     compile_host_plan(p, carrier, policy);
     HostLifted& h = p.lifted;
     h.v0 = m.initial_variance;
@@ -570,6 +578,15 @@ void compile_rough_plan(HostPlan& p, const pe_request& carrier, const pe_lifted_
     // RQMC dimensions of the plan's own grid: two normals per step
     const size_t steps = compile_time_grid(p.rough_events, maximum_step).size() - 1;
     compile_diagnostics(p, policy, static_cast<uint32_t>(2 * steps));
+    if (american) {
+        const pe_engine& te = p.req.lsm.training_engine;
+        p.train_diag = pe_mc_diagnostics{};
+        if (te.kind == PE_ENGINE_RQMC) {
+            p.train_diag.has_checksums = 1;
+            rqmc_checksums(te.seed, te.scrambles, static_cast<uint32_t>(2 * steps), p.train_diag.direction_checksum,
+                           p.train_diag.scramble_checksum);
+        }
+    }
 }
 
 // ------------------------------------------------------------------ plan --

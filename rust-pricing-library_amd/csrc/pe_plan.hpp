@@ -135,6 +135,10 @@ struct LsmLaunch;
 struct LsmLaunchDeleter {
     void operator()(LsmLaunch* l) const;
 };
+struct RoughLsmLaunch;
+struct RoughLsmLaunchDeleter {
+    void operator()(RoughLsmLaunch* l) const;
+};
 
 // A graph product's tape for the BS batch kernel (compile_graph, pe_graph.cpp).
 struct GraphTape {
@@ -211,6 +215,7 @@ struct HostPlan {
     HostLifted lifted;
     double rough_max_step = 0.0;
     std::vector<double> rough_events;
+    std::unique_ptr<RoughLsmLaunch, RoughLsmLaunchDeleter> rough_lsm_launch;   // Bermudan under lifted Heston
 
     // BS/Black-76 trade descriptor and observation table for the batch kernel
     BsTrade trade{};
@@ -282,6 +287,10 @@ hipError_t launch_rough_f64(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 
                             unsigned long long* error, hipStream_t s);
 hipError_t launch_rough_f32(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 grid, uint32_t payoff_items,
                             unsigned long long* error, hipStream_t s);
+hipError_t launch_rough_lsm_f64(const RoughLsmArgs& a, dim3 train_grid, dim3 value_grid, hipStream_t s,
+                               hipEvent_t paths_done, hipEvent_t between);
+hipError_t launch_rough_lsm_f32(const RoughLsmArgs& a, dim3 train_grid, dim3 value_grid, hipStream_t s,
+                               hipEvent_t paths_done, hipEvent_t between);
 hipError_t launch_lsm_dual_f64(const LsmDualArgs& da, uint64_t pairs, uint32_t outer_blocks, uint32_t basis,
                                hipStream_t s);
 hipError_t launch_lv_risk_f64(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s);

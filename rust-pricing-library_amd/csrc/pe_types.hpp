@@ -241,6 +241,7 @@ struct RoughArgs {
     double v0, kappa, theta, nu, rho, rho_bar;
     double weights[kMaxRoughFactors];
     uint32_t n_steps, n_factors, n_store, rqmc, antithetic;
+    uint32_t domain;              // Philox domain of the normals (Valuation 0, LsmTrain 1)
 };
 
 struct RoughProduct {    // terminal payoff read from one store slot
@@ -263,6 +264,57 @@ struct RoughPayoffArgs {
     void* partials;
     uint64_t units, total_paths;
     uint32_t antithetic;
+    uint32_t pad;
+};
+
+// ------------------------------------------ early exercise on lifted Heston --
+// Regression on the features (spot, variance): standardized monomials of
+// total degree <= 2 in the order 1, s, v, s^2, s v, v^2.
+constexpr int kRlCols = 6;
+constexpr int kRlGram = kRlCols * (kRlCols + 1) / 2 + kRlCols + 2;   // packed Gram, X'y, y'y, count
+constexpr int kRlStats = 8;                                          // per-pass partial width (stats and variance)
+
+struct RoughLsmDecision {
+    double coef[kRlCols];          // per basis column, original order (0 for excluded columns)
+    double diag[kRlCols];          // |R_pp| in pivot order
+    double mean[2], variance[2], scale[2], inv_scale[2], zero_threshold[2];
+    double rank_threshold, rss;
+    uint32_t pivot[kRlCols];       // original column index per pivot position (active columns)
+    uint32_t rank, continue_all, itm_rows, inactive;   // inactive: bit f set when feature f is inactive
+    uint32_t active;               // number of active columns (the pivot entries in use)
+    uint32_t pad;
+};
+
+// Per-date feature statistics of the in-the-money training rows.
+struct RoughLsmScaling {
+    double itm;                    // row count
+    double mean[2], feature_scale[2];
+    double variance[2], scale[2], zero_threshold[2];
+    uint32_t inactive, pad;
+};
+
+struct RoughLsmArgs {
+    RoughArgs train, value;        // the training and valuation engines on one grid and model
+    const uint32_t* ex_nodes;      // grid node of each exercise date
+    const double* ex_discount;     // DF(date)
+    const double* ex_scale;        // physical spot per unit of F at the date's node
+    double* store_s;               // training spots [date][path]
+    double* store_v;               // training variances [date][path]
+    double* cash;                  // training cash flow x DF(stopping date) [path]
+    double* partials;              // [workgroup][kRlGram]
+    RoughLsmScaling* scalings;     // [date]
+    RoughLsmDecision* decisions;   // [non-terminal date]
+    void* value_partials;          // BlockStat per valuation item
+    unsigned long long* counts;    // valuation exercise counts [date]
+    uint32_t* stops;               // optional: exercise-date index per valuation trajectory
+    double* in_sample;             // mean of the training cash flows under the trained policy
+    unsigned long long* error;
+    uint64_t train_paths;          // training trajectories (units x scrambles x passes)
+    uint64_t value_chunk;          // valuation units per item
+    double strike, notional, itm_tol, abs_tol, rel_tol;
+    uint32_t n_dates, call, columns;   // columns: 3 (degree 1) or 6 (degree 2)
+    uint32_t n_wg;                 // workgroups of the training passes
+    uint32_t value_items;
     uint32_t pad;
 };
 

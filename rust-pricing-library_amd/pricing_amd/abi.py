@@ -312,20 +312,26 @@ class pe_lsm_decision(C.Structure):
         ("continue_all", C.c_int32),
         ("rank", C.c_uint32),
         ("itm_rows", C.c_uint64),
-        ("coefficients", C.c_double * 4),
-        ("diagonal_abs", C.c_double * 4),
-        ("pivot_order", C.c_uint32 * 4),
+        ("coefficients", C.c_double * 6),
+        ("diagonal_abs", C.c_double * 6),
+        ("pivot_order", C.c_uint32 * 6),
         ("feature_mean", C.c_double),
         ("feature_population_variance", C.c_double),
         ("feature_scale", C.c_double),
         ("feature_inactive", C.c_int32),
-        ("pad", C.c_int32),
+        ("active_columns", C.c_uint32),
         ("rank_threshold", C.c_double),
         ("residual_sum_squares", C.c_double),
         ("region_left", C.c_int32),
         ("region_cut_count", C.c_uint32),
         ("region_cuts", C.c_float * 64),
         ("zero_scale_threshold", C.c_double),
+        ("variance_mean", C.c_double),
+        ("variance_population_variance", C.c_double),
+        ("variance_scale", C.c_double),
+        ("variance_zero_scale_threshold", C.c_double),
+        ("variance_inactive", C.c_int32),
+        ("feature_count", C.c_uint32),
     ]
 
 

@@ -1,0 +1,167 @@
+"""GPU parity of the Bermudan put under lifted Heston (least squares on the
+features spot and variance state, in-the-money rows and exercise).
+
+* FP64 device mode against the oracle's evaluate_rough_american on the smile
+  model (H = 0.1, 20 factors, step 2/256): the price within 1e-12, the
+  standard error within 1e-10, the in-sample value within 1e-12, every
+  date's exercise count and every valuation trajectory's stopping index
+  exactly, and each date's regression -- in-the-money rows, rank and pivot
+  order exactly; the feature means and population variances to 1e-12, the
+  coefficients of the standardized monomials to 1e-9, |R_pp| to 1e-10 and
+  the residual sum of squares to 1e-8 (an FP64 Gram + pivoted Cholesky
+  against the oracle's Householder CPQR) -- for pseudo-random and RQMC
+  engines, with and without antithetic pairs, total degree 1 and 2.  At
+  degree 1 the constant and both standardized linear columns have squared
+  norm exactly n (the standardization makes sum s^2 = n), a tie that the
+  reference's exact comparison resolves by rounding, so the pivot order is
+  compared as a set there; |R_pp| and the coefficients still match.
+* The constant-variance limit: the variance feature is inactive and its
+  columns are excluded on the device as in the oracle.
+* The mixed mode (FP32 valuation paths, FP64 training) within 0.1 standard
+  errors of the FP64 mode at 2^18 valuation paths.
+* pe_plan_evaluate and pe_batch_evaluate give the pe_lsm_evaluate price; the
+  report's engines, sizes and RQMC checksums describe the plan.
+"""
+import math
+
+import pytest
+
+from _support import oracle_rough_lsm
+from pricing_amd import abi, engine, workloads
+
+pytestmark = pytest.mark.gpu
+
+STEP = 2.0 / 256
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _init():
+    engine.init(0)
+
+
+def _model():
+    return workloads.lifted_heston_model(engine.lifted_from_rough(0.1, 20, 2.5))
+
+
+def _close(a, b, rel):
+    return abs(a - b) <= rel * max(abs(a), abs(b)) + 1e-300
+
+
+def _device(req, model, backend="fp64", stops=0):
+    plan = engine.RoughPlan.compile(req, model, STEP)
+    res, rep = engine.evaluate_lsm([plan], backend, stopping_capacity=stops)
+    return plan, res[0], rep[0]
+
+
+def _compare_decisions(rep, rows, pivots_as_set=False):
+    assert len(rep.decisions) == len(rows)
+    for d, (dec, r) in enumerate(zip(rep.decisions, rows)):
+        assert dec.feature_count == 2
+        assert dec.itm_rows == int(r[0]) and bool(dec.continue_all) == bool(r[26]), d
+        if dec.continue_all:
+            continue
+        assert dec.rank == int(r[1]), (d, dec.rank, r[1])
+        assert _close(dec.rank_threshold, r[2], 1e-10), d
+        assert _close(dec.residual_sum_squares, r[3], 1e-8), (d, dec.residual_sum_squares, r[3])
+        assert _close(dec.feature_mean, r[4], 1e-12) and _close(dec.feature_population_variance, r[5], 1e-12), d
+        assert bool(dec.feature_inactive) == bool(r[27]) and bool(dec.variance_inactive) == bool(r[28]), d
+        if not dec.variance_inactive:
+            assert _close(dec.variance_mean, r[6], 1e-12), d
+            assert _close(dec.variance_population_variance, r[7], 1e-12), d
+        for k in range(6):
+            assert abs(dec.coefficients[k] - r[8 + k]) <= 1e-9 * max(1.0, abs(r[8 + k])), (d, k, dec.coefficients[k],
+                                                                                           r[8 + k])
+        for k in range(dec.active_columns):
+            assert _close(dec.diagonal_abs[k], r[14 + k], 1e-10), (d, k)
+        got = [dec.pivot_order[k] for k in range(dec.active_columns)]
+        want = [int(r[20 + k]) for k in range(dec.active_columns)]
+        assert (sorted(got) == sorted(want)) if pivots_as_set else (got == want), (d, got, want)
+
+
+CASES = [
+    dict(units=1 << 14, train_units=1 << 14),
+    dict(units=1 << 13, train_units=1 << 14, antithetic=True),
+    dict(units=1 << 12, train_units=1 << 12, scrambles=4),
+    dict(units=1 << 11, train_units=1 << 12, scrambles=4, antithetic=True),
+    dict(units=1 << 14, train_units=1 << 14, max_degree=1),
+]
+
+
+@pytest.mark.parametrize("kw", CASES, ids=["pseudo", "antithetic", "rqmc", "rqmc-antithetic", "degree1"])
+def test_fp64_matches_oracle(kw):
+    model = _model()
+    req = workloads.rough_bermudan(**kw)
+    scr = kw.get("scrambles", 0) or 1
+    traj = kw["units"] * scr * (2 if kw.get("antithetic") else 1)
+    _, res, rep = _device(req, model, stops=traj)
+    ref, rows, ins, counts, stops = oracle_rough_lsm(req, model, STEP, workers=8, stop_capacity=traj)
+    g, o = res.estimates[0], ref.estimates[0]
+    assert _close(g.value, o.value, 1e-12), (g.value, o.value)
+    assert _close(g.standard_error, o.standard_error, 1e-10), (g.standard_error, o.standard_error)
+    assert res.evaluated_paths == ref.evaluated_paths and res.time_steps == ref.time_steps
+    assert _close(rep.in_sample_value, ins, 1e-12), (rep.in_sample_value, ins)
+    assert rep.exercise_counts == counts[:rep.exercise_dates]
+    assert rep.stopping_indices == stops
+    _compare_decisions(rep, rows, pivots_as_set=kw.get("max_degree", 2) == 1)
+    assert sum(rep.exercise_counts[:-1]) > 0
+
+
+def test_constant_variance_limit_excludes_variance_columns():
+    v0 = 0.04
+    w, x = engine.lifted_from_rough(0.1, 20, 2.5)
+    model = abi.lifted_heston(v0, 0.0, v0, 0.0, -0.7, w, x)
+    req = workloads.rough_bermudan(units=1 << 13, train_units=1 << 13, dates=5)
+    _, res, rep = _device(req, model)
+    ref, rows, ins, counts, _ = oracle_rough_lsm(req, model, STEP, workers=8)
+    assert _close(res.estimates[0].value, ref.estimates[0].value, 1e-12)
+    for dec in rep.decisions:
+        assert dec.variance_inactive and not dec.feature_inactive and dec.active_columns == 3
+        assert all(dec.coefficients[k] == 0.0 for k in (2, 4, 5))
+    _compare_decisions(rep, rows)
+
+
+def test_report_describes_the_plan():
+    model = _model()
+    req = workloads.rough_bermudan(units=1 << 10, train_units=1 << 11, scrambles=4, dates=6)
+    _, res, rep = _device(req, model)
+    assert rep.exercise_dates == 6 and len(rep.decisions) == 5
+    assert rep.policy_feature_count == 2 and rep.policy_max_degree == 2
+    assert rep.training_random_domain == rep.valuation_random_domain == abi.PE_DOMAIN_RQMC_SCRAMBLE
+    assert rep.training_sampling_units == rep.valuation_sampling_units == 4
+    assert rep.training_trajectories == 4 << 11 and rep.valuation_trajectories == 4 << 10
+    assert rep.training_direction_checksum and rep.valuation_direction_checksum
+    assert rep.training_scramble_checksum != rep.valuation_scramble_checksum   # different seeds
+    assert sum(rep.exercise_counts) == rep.valuation_trajectories
+    assert abs(sum(rep.exercise_probabilities) - 1.0) < 1e-12
+    assert res.sobol_dimension == 2 * res.time_steps
+    assert rep.training_ms > 0.0 and rep.valuation_ms > 0.0
+
+
+def test_entry_points_agree():
+    model = _model()
+    req = workloads.rough_bermudan(units=1 << 12, train_units=1 << 12)
+    plan, res, _ = _device(req, model)
+    single = plan.evaluate("fp64")
+    batch = engine.evaluate_batch([plan], "fp64")[0]
+    for r in (single, batch):
+        assert r.estimates[0].value == res.estimates[0].value
+        assert r.estimates[0].standard_error == res.estimates[0].standard_error
+    again = plan.evaluate("fp64")
+    assert again.estimates[0].value == res.estimates[0].value   # deterministic relaunch
+
+
+def test_mixed_within_a_tenth_of_a_standard_error():
+    model = _model()
+    req = workloads.rough_bermudan(units=1 << 18, train_units=1 << 16)
+    plan = engine.RoughPlan.compile(req, model, STEP)
+    f64, rep64 = engine.evaluate_lsm([plan], "fp64")
+    mix, repmx = engine.evaluate_lsm([plan], "mixed")
+    a, b = f64[0].estimates[0], mix[0].estimates[0]
+    gap = abs(a.value - b.value) / a.standard_error
+    print(f"rough Bermudan fp64 {a.value:.6f} mixed {b.value:.6f} se {a.standard_error:.2e} gap {gap:.4f} SE")
+    assert gap < 0.1, gap
+    # the policy is trained on FP64 paths in both modes: identical decisions
+    for d64, dmx in zip(rep64[0].decisions, repmx[0].decisions):
+        assert list(d64.coefficients) == list(dmx.coefficients)
+    assert rep64[0].in_sample_value == repmx[0].in_sample_value
+    assert math.isfinite(b.standard_error) and abs(b.standard_error / a.standard_error - 1.0) < 0.01
