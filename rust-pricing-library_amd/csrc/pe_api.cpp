@@ -912,8 +912,9 @@ void prepare_rough_lsm(HostPlan& p, pe_backend backend) {
     a.store_s = L->store_s.as<double>();
     a.store_v = L->store_v.as<double>();
     a.cash = L->cash.as<double>();
-    // training passes: about 1024 paths per workgroup, at most 4096 workgroups
-    a.n_wg = static_cast<uint32_t>(std::min<uint64_t>(4096, std::max<uint64_t>(1, (a.train_paths + 1023) / 1024)));
+    // training passes: about 512 paths per workgroup (two per lane), at most
+    // 4096 workgroups
+    a.n_wg = static_cast<uint32_t>(std::min<uint64_t>(4096, std::max<uint64_t>(1, (a.train_paths + 511) / 512)));
     L->partials.reserve(static_cast<size_t>(a.n_wg) * kRlGram * sizeof(double));
     a.partials = L->partials.as<double>();
     L->scalings.reserve(std::max<size_t>(1, dates) * sizeof(RoughLsmScaling));
