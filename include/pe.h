@@ -508,6 +508,21 @@ typedef struct pe_lsm_report {
     uint32_t policy_feature_count, policy_max_degree;   /* PolynomialBasisSpec */
     double itm_abs_tolerance, abs_rank_tolerance, rel_rank_tolerance;
     uint64_t max_matrix_elements;
+    /* Network policy (PE_LSM_MLP) only; no reference counterpart.  The
+     * weighted-Laguerre polynomial policy is trained on the same training
+     * paths and valued on the same valuation paths as the network: it is the
+     * judge, and the fallback when its out-of-sample value is the higher one
+     * (fallback_used = 1: the result, exercise counts and probabilities are
+     * the polynomial policy's).  Stopping indices and the decisions' region
+     * cuts are the network's; the decisions' coefficients are the judge's. */
+    pe_estimate network_price;      /* out-of-sample value of the network policy */
+    pe_estimate judge_price;        /* out-of-sample value of the polynomial policy */
+    double judge_in_sample_value;
+    int32_t fallback_used;
+    uint32_t pad;
+    double network_training_ms;     /* training paths + network induction (HIP events) */
+    double judge_training_ms;       /* the polynomial judge's induction on the same paths */
+    double network_valuation_ms, judge_valuation_ms;
 } pe_lsm_report;
 
 /* Prices `count` American plans by least-squares Monte Carlo
@@ -518,7 +533,9 @@ typedef struct pe_lsm_report {
  * pe_plan_evaluate returns for plans[i]; reports (optional, `count` entries)
  * receive the policy diagnostics.  The method of each plan's pe_lsm selects
  * the reference's monomial CPQR policy (solved by normal equations with its
- * pivot and rank rules), the weighted-Laguerre policy or the MLP policy. */
+ * pivot and rank rules), the weighted-Laguerre policy or the MLP policy (one
+ * network per trade, judged against and falling back to the weighted-Laguerre
+ * policy; price-only: fixed-policy Greeks are PE_ERR_UNSUPPORTED). */
 pe_status pe_lsm_evaluate(const pe_plan* const* plans, uint32_t count, pe_backend backend, pe_result* results,
                           pe_lsm_report* reports);
 

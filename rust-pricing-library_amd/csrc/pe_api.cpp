@@ -1373,15 +1373,20 @@ struct LsmLaunch {
     std::vector<unsigned long long> host_counts;
     std::vector<double> host_in_sample;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};   // start, training done, valuation done, paths done
-    // network policy
+    hipEvent_t mlp_ev[2] = {nullptr, nullptr};                 // network trained, network valued
+    // network policy and its polynomial judge
     bool mlp = false;
-    uint32_t mlp_wgs = 0;
     MlpLsmArgs mlp_args{};
-    std::vector<float> mlp_init;
     std::vector<LsmCuts> host_cuts;
-    DeviceBuffer weights, adam_m, adam_v, mlp_partials, mlp_rows, mlp_stats, cuts, mlp_err;
+    std::vector<BlockStat> host_judge_partials;
+    std::vector<unsigned long long> host_judge_counts;
+    std::vector<double> host_judge_in_sample;
+    DeviceBuffer mlp_init, weights, adam_m, adam_v, mlp_rows, cuts, mlp_err, judge_cash, judge_partials, judge_counts,
+        judge_in_sample;
     ~LsmLaunch() {
         for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t e : mlp_ev)
             if (e) (void)hipEventDestroy(e);
     }
 };
@@ -1390,35 +1395,31 @@ void LsmLaunchDeleter::operator()(LsmLaunch* l) const { delete l; }
 
 namespace {
 
-// The network policy shares one network per exercise date across the
-// launch: every plan must have the same exercise days, side, training engine
-// shape and network settings.  He-uniform initial weights from SplitMix64 of
-// the network seed; Adam state starts at zero for every evaluation.
-void prepare_mlp(LsmLaunch& L, std::vector<HostPlan*>& plans) {
+// The network policy trains one network per trade (one workgroup each),
+// from He-uniform hidden-layer weights drawn from SplitMix64 of the network
+// seed and a zero output layer (every trade starts from the same vector;
+// Adam state starts at zero for every evaluation).  The weighted-Laguerre judge shares the training paths
+// and the valuation paths.
+void prepare_mlp(LsmLaunch& L, std::vector<HostPlan*>& plans, size_t cash_bytes) {
     const HostPlan& p0 = *plans[0];
     const pe_lsm& l = p0.req.lsm;
     for (const HostPlan* q : plans) {
         const pe_lsm& o = q->req.lsm;
-        if (q->obs_days != p0.obs_days || q->req.product.side != p0.req.product.side ||
-            q->lsm_trade.train_paths != p0.lsm_trade.train_paths ||
-            q->lsm_trade.train.antithetic != p0.lsm_trade.train.antithetic || o.mlp_hidden != l.mlp_hidden ||
-            o.mlp_epochs != l.mlp_epochs || o.mlp_batch != l.mlp_batch || o.mlp_learning_rate != l.mlp_learning_rate ||
-            o.mlp_seed != l.mlp_seed)
-            throw PlanError(PE_ERR_INVALID_ARGUMENT,
-                            "plans sharing the MLP policy need one exercise schedule, side, training shape and network");
+        if (o.mlp_hidden != l.mlp_hidden || o.mlp_epochs != l.mlp_epochs || o.mlp_batch != l.mlp_batch ||
+            o.mlp_learning_rate != l.mlp_learning_rate || o.mlp_seed != l.mlp_seed)
+            throw PlanError(PE_ERR_INVALID_ARGUMENT, "plans of one network-policy launch need one network setting");
+        if (q->risk) throw PlanError(PE_ERR_UNSUPPORTED, "fixed-policy Greeks with the network policy");
+        if (q->req.product.side != p0.req.product.side)
+            throw PlanError(PE_ERR_INVALID_ARGUMENT, "plans of one network-policy launch need one option side");
     }
     if (l.mlp_hidden != 64) throw PlanError(PE_ERR_UNSUPPORTED, "MLP hidden width must be 64 on the device");
     if (l.mlp_epochs == 0 || !(l.mlp_learning_rate > 0.0)) throw PlanError(PE_ERR_INVALID_ARGUMENT, "MLP epochs / rate");
     const uint32_t B = l.mlp_batch;
-    // 16 workgroups of eight waves per step (the grid barrier costs grow with
-    // the workgroup count); each wave takes 64 rows of an 8192-row batch
-    const uint32_t G = std::min<uint32_t>(16, B / 512);
-    if (G == 0 || (B / G) % 128 != 0 || (B / G) * G != B || B / G > 2048)
-        throw PlanError(PE_ERR_INVALID_ARGUMENT, "MLP batch must be a multiple of 512, at most 32768");
+    if (B == 0 || B % 128 != 0 || B > (1u << 20))
+        throw PlanError(PE_ERR_INVALID_ARGUMENT, "MLP batch must be a positive multiple of 128, at most 2^20");
     L.mlp = true;
-    L.mlp_wgs = G;
-    // He-uniform initial weights
-    L.mlp_init.assign(kMlpParameterCount, 0.0f);
+    for (hipEvent_t& e : L.mlp_ev) check(hipEventCreate(&e), "hipEventCreate");
+    std::vector<float> init(kMlpParameterCount, 0.0f);
     uint64_t state = l.mlp_seed;
     auto uniform = [&]() {
         state += 0x9E3779B97F4A7C15ull;
@@ -1428,41 +1429,48 @@ void prepare_mlp(LsmLaunch& L, std::vector<HostPlan*>& plans) {
         z ^= z >> 31;
         return 2.0 * (static_cast<double>(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
     };
-    for (int k = 0; k < 128; ++k) L.mlp_init[k] = static_cast<float>(uniform() * std::sqrt(6.0 / 2.0));
-    for (int k = 192; k < 4288; ++k) L.mlp_init[k] = static_cast<float>(uniform() * std::sqrt(6.0 / 64.0));
-    for (int k = 4352; k < 4416; ++k) L.mlp_init[k] = static_cast<float>(uniform() * std::sqrt(3.0 / 64.0));
-    const uint64_t pool = static_cast<uint64_t>(plans.size()) * p0.lsm_trade.train_paths;
-    L.weights.reserve(kMlpParameterCount * sizeof(float));
-    L.adam_m.reserve(kMlpParameterCount * sizeof(float));
-    L.adam_v.reserve(kMlpParameterCount * sizeof(float));
-    L.mlp_partials.reserve(static_cast<size_t>(G) * (kMlpParameterCount + 1) * sizeof(float));
-    L.mlp_rows.reserve(static_cast<size_t>(pool) * 3 * sizeof(float) + 16);
-    L.mlp_stats.reserve(static_cast<size_t>(G) * 8 * sizeof(double));
+    for (int k = 0; k < 128; ++k) init[k] = static_cast<float>(uniform() * std::sqrt(6.0 / 2.0));
+    for (int k = 192; k < 4288; ++k) init[k] = static_cast<float>(uniform() * std::sqrt(6.0 / 64.0));
+    // the output layer starts at zero: the untrained network is the control
+    // variate alone (continuation = the European value + the mean residual)
+    L.mlp_init.upload(init);
+    const size_t n = plans.size();
+    const size_t stride = (kMlpParameterCount + 3) / 4 * 4;
+    uint64_t rows = 0;
+    for (const HostPlan* q : plans) rows += q->lsm_trade.train_paths;
+    L.weights.reserve(n * stride * sizeof(float));
+    L.adam_m.reserve(n * stride * sizeof(float));
+    L.adam_v.reserve(n * stride * sizeof(float));
+    L.mlp_rows.reserve(static_cast<size_t>(rows) * 4 * sizeof(float) + 16);
     L.cuts.reserve(std::max<size_t>(1, L.n_decisions) * sizeof(LsmCuts));
     L.mlp_err.reserve(sizeof(unsigned int));
     L.host_cuts.resize(L.n_decisions);
+    L.judge_cash.reserve(cash_bytes + 256);
+    L.judge_partials.reserve(std::max<size_t>(1, L.n_partials) * sizeof(BlockStat));
+    L.judge_counts.reserve(L.n_counts * sizeof(unsigned long long));
+    L.judge_in_sample.reserve(n * sizeof(double));
+    L.host_judge_partials.resize(L.n_partials);
+    L.host_judge_counts.resize(L.n_counts);
+    L.host_judge_in_sample.resize(n);
     MlpLsmArgs& m = L.mlp_args;
+    m.init = L.mlp_init.as<float>();
     m.weights = L.weights.as<float>();
     m.adam_m = L.adam_m.as<float>();
     m.adam_v = L.adam_v.as<float>();
-    m.partials = L.mlp_partials.as<float>();
-    m.rows = L.mlp_rows.as<float>();
-    m.stats = L.mlp_stats.as<double>();
+    m.rows = L.mlp_rows.as<float4>();
     m.cuts = L.cuts.as<LsmCuts>();
     m.error = L.mlp_err.as<unsigned int>();
-    m.n_trades = static_cast<uint32_t>(plans.size());
-    m.n_dates = p0.lsm_trade.n_dates;
-    m.paths_per_trade = static_cast<uint32_t>(p0.lsm_trade.train_paths);
+    m.cash_bytes = cash_bytes;
+    m.n_trades = static_cast<uint32_t>(n);
     m.batch = B;
     m.epochs = l.mlp_epochs;
-    m.rows_per_wg = B / G;
     m.lr = static_cast<float>(l.mlp_learning_rate);
     m.beta1 = 0.9f;
     m.beta2 = 0.999f;
     m.eps = 1e-8f;
-    m.call = p0.req.product.side == PE_CALL ? 1u : 0u;
-    m.grid_lo = m.call ? 1.0f : 0.25f;
-    m.grid_hi = m.call ? 4.0f : 1.0f;
+    const bool call = p0.req.product.side == PE_CALL;
+    m.grid_lo = call ? 1.0f : 0.25f;
+    m.grid_hi = call ? 4.0f : 1.0f;
 }
 
 LsmLaunch& prepare_lsm(std::vector<HostPlan*>& plans, pe_backend backend) {
@@ -1567,7 +1575,7 @@ LsmLaunch& prepare_lsm(std::vector<HostPlan*>& plans, pe_backend backend) {
     L->host_counts.resize(L->n_counts);
     L->host_in_sample.resize(plans.size());
     for (hipEvent_t& e : L->ev) check(hipEventCreate(&e), "hipEventCreate");
-    if (L->basis == kLsmMlp) prepare_mlp(*L, plans);
+    if (L->basis == kLsmMlp) prepare_mlp(*L, plans, static_cast<size_t>(cash_elems) * real);
     p0.lsm_launch.reset(L.release());
     return *p0.lsm_launch;
 }
@@ -1684,24 +1692,29 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
     train.n_items = static_cast<uint32_t>(L.train_items.size());
     value.items = L.d_value_items.as<LsmItem>();
     value.n_items = static_cast<uint32_t>(L.value_items.size());
+    LsmArgs judge = train, judge_value = value;
     if (L.mlp) {
         train.cuts = value.cuts = L.cuts.as<LsmCuts>();
         L.mlp_args.lsm = train;
-        check(hipMemcpyAsync(L.weights.ptr, L.mlp_init.data(), kMlpParameterCount * sizeof(float),
-                             hipMemcpyHostToDevice, t_stream),
-              "MLP weights");
-        check(hipMemsetAsync(L.adam_m.ptr, 0, kMlpParameterCount * sizeof(float), t_stream), "Adam state");
-        check(hipMemsetAsync(L.adam_v.ptr, 0, kMlpParameterCount * sizeof(float), t_stream), "Adam state");
+        judge.cash = L.judge_cash.ptr;
+        judge.in_sample = L.judge_in_sample.as<double>();
+        judge_value.partials = L.judge_partials.ptr;
+        judge_value.counts = L.judge_counts.as<unsigned long long>();
+        judge_value.stops = nullptr;
+        judge_value.risk_partials = nullptr;
         check(hipMemsetAsync(L.mlp_err.ptr, 0, sizeof(unsigned int), t_stream), "MLP error word");
         check(hipMemsetAsync(L.cuts.ptr, 0, std::max<size_t>(1, L.n_decisions) * sizeof(LsmCuts), t_stream), "cuts");
+        check(hipMemsetAsync(L.judge_counts.ptr, 0, L.n_counts * sizeof(unsigned long long), t_stream), "counts");
     }
     check(hipEventRecord(L.ev[0], t_stream), "hipEventRecord");
+    // network launch marks: paths, network trained, judge trained, network valued
+    const hipEvent_t mlp_marks[4] = {L.ev[3], L.mlp_ev[0], L.ev[1], L.mlp_ev[1]};
     if (L.mlp)
         check(backend == PE_BACKEND_GPU_FP64
-                  ? launch_lsm_mlp_f64(train, train.n_items, L.mlp_args, L.mlp_wgs, value, value.n_items, L.risk, t_stream,
-                                       L.ev[3], L.ev[1])
-                  : launch_lsm_mlp_f32(train, train.n_items, L.mlp_args, L.mlp_wgs, value, value.n_items, L.risk, t_stream,
-                                       L.ev[3], L.ev[1]),
+                  ? launch_lsm_mlp_f64(train, train.n_items, L.mlp_args, judge, value, judge_value, value.n_items,
+                                       L.risk, t_stream, mlp_marks)
+                  : launch_lsm_mlp_f32(train, train.n_items, L.mlp_args, judge, value, judge_value, value.n_items,
+                                       L.risk, t_stream, mlp_marks),
               "MLP LSM launch");
     else if (backend == PE_BACKEND_GPU_FP64)
         check(launch_lsm_f64(train, train.n_items, value, value.n_items, L.basis, L.risk, t_stream, L.ev[3], L.ev[1]),
@@ -1718,6 +1731,14 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
     check(hipEventElapsedTime(&train_ms, L.ev[0], L.ev[1]), "hipEventElapsedTime");
     check(hipEventElapsedTime(&path_ms, L.ev[0], L.ev[3]), "hipEventElapsedTime");
     check(hipEventElapsedTime(&value_ms, L.ev[1], L.ev[2]), "hipEventElapsedTime");
+    // network policy: its own training (paths + network) and valuation, and the judge's
+    float nn_train_ms = 0.0f, judge_train_ms = 0.0f, nn_value_ms = 0.0f, judge_value_ms = 0.0f;
+    if (L.mlp) {
+        check(hipEventElapsedTime(&nn_train_ms, L.ev[0], L.mlp_ev[0]), "hipEventElapsedTime");
+        check(hipEventElapsedTime(&judge_train_ms, L.mlp_ev[0], L.ev[1]), "hipEventElapsedTime");
+        check(hipEventElapsedTime(&nn_value_ms, L.ev[1], L.mlp_ev[1]), "hipEventElapsedTime");
+        check(hipEventElapsedTime(&judge_value_ms, L.mlp_ev[1], L.ev[2]), "hipEventElapsedTime");
+    }
     check(copy_on_stream(L.host_partials.data(), L.partials.ptr, L.n_partials * sizeof(BlockStat), hipMemcpyDeviceToHost),
           "partials copy");
     if (L.risk)
@@ -1747,6 +1768,15 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
         if (L.n_decisions)
             check(copy_on_stream(L.host_cuts.data(), L.cuts.ptr, L.n_decisions * sizeof(LsmCuts), hipMemcpyDeviceToHost),
                   "cuts copy");
+        check(copy_on_stream(L.host_judge_partials.data(), L.judge_partials.ptr, L.n_partials * sizeof(BlockStat),
+                             hipMemcpyDeviceToHost),
+              "judge partials copy");
+        check(copy_on_stream(L.host_judge_counts.data(), L.judge_counts.ptr, L.n_counts * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost),
+              "judge counts copy");
+        check(copy_on_stream(L.host_judge_in_sample.data(), L.judge_in_sample.ptr, plans.size() * sizeof(double),
+                             hipMemcpyDeviceToHost),
+              "judge in-sample copy");
     }
     for (size_t ti = 0; ti < plans.size(); ++ti) {
         const HostPlan& p = *plans[ti];
@@ -1757,6 +1787,19 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
         pe_result* out = outs + ti;
         std::memset(out, 0, sizeof(*out));
         assemble(p, mine, L.chunks[ti], L.scrambles[ti], 1, 0, 1, out);
+        // network policy: the polynomial judge's out-of-sample value on the
+        // same valuation paths; the higher of the two is the price
+        pe_result judge{};
+        bool fallback = false;
+        pe_estimate network = out->estimates[0];
+        if (L.mlp) {
+            std::vector<BlockStat> jm(L.host_judge_partials.begin() + L.first_partial[ti],
+                                      L.host_judge_partials.begin() + L.first_partial[ti] + per);
+            assemble(p, jm, L.chunks[ti], L.scrambles[ti], 1, 0, 1, &judge);
+            fallback = judge.estimates[0].value > out->estimates[0].value;
+            if (fallback) *out = judge;
+        }
+        const std::vector<unsigned long long>& counts = fallback ? L.host_judge_counts : L.host_counts;
         if (p.risk) {
             // fixed-policy Greeks: components 1..9 from the valuation kernel's risk partials
             // (execute_early_exercise, crates/pricing/src/engine/risk/early_exercise.rs:115-170)
@@ -1802,6 +1845,20 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
         r.abs_rank_tolerance = p.req.lsm.abs_rank_tolerance;
         r.rel_rank_tolerance = p.req.lsm.rel_rank_tolerance;
         r.max_matrix_elements = p.req.lsm.max_matrix_elements;
+        r.network_price = r.judge_price = pe_estimate{};
+        r.judge_in_sample_value = 0.0;
+        r.fallback_used = 0;
+        r.network_training_ms = r.judge_training_ms = r.network_valuation_ms = r.judge_valuation_ms = 0.0;
+        if (L.mlp) {
+            r.network_price = network;
+            r.judge_price = judge.estimates[0];
+            r.judge_in_sample_value = L.host_judge_in_sample[ti];
+            r.fallback_used = fallback ? 1 : 0;
+            r.network_training_ms = nn_train_ms;
+            r.judge_training_ms = judge_train_ms;
+            r.network_valuation_ms = nn_value_ms;
+            r.judge_valuation_ms = judge_value_ms;
+        }
         if (r.stopping_indices && r.stopping_capacity) {
             const uint64_t k = std::min<uint64_t>(r.stopping_capacity, vtraj);
             check(copy_on_stream(r.stopping_indices, L.stops.as<uint32_t>() + t.stop_offset, k * sizeof(uint32_t),
@@ -1811,9 +1868,9 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
         for (uint32_t d = 0; d < t.n_dates && d < r.capacity; ++d) {
             if (r.exercise_probabilities)
                 r.exercise_probabilities[d] =
-                    static_cast<double>(L.host_counts[t.count_offset + d]) / static_cast<double>(vtraj);
+                    static_cast<double>(counts[t.count_offset + d]) / static_cast<double>(vtraj);
             if (r.dividend_collisions) r.dividend_collisions[d] = p.lsm_collisions[d];
-            if (r.exercise_counts) r.exercise_counts[d] = L.host_counts[t.count_offset + d];
+            if (r.exercise_counts) r.exercise_counts[d] = counts[t.count_offset + d];
             if (!r.decisions || d + 1 >= t.n_dates) continue;
             const LsmDecision& s = L.host_decisions[t.decision_offset + d];
             pe_lsm_decision& o = r.decisions[d];
@@ -1823,7 +1880,6 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
                 o.region_left = static_cast<int32_t>(c.left);
                 o.region_cut_count = c.count;
                 for (uint32_t k = 0; k < c.count && k < 64; ++k) o.region_cuts[k] = c.cut[k];
-                continue;
             }
             o.continue_all = s.continue_all ? 1 : 0;
             o.itm_rows = s.itm_rows;

@@ -105,9 +105,10 @@ hipError_t launch_mlp_forward(const float* params, const float* x, uint32_t rows
     return hipGetLastError();
 }
 
-hipError_t launch_lsm_mlp_f32(const LsmArgs& train, uint32_t train_items, const MlpLsmArgs& m, uint32_t wgs,
-                              const LsmArgs& value, uint32_t value_items, bool risk, hipStream_t s, hipEvent_t paths_done, hipEvent_t between) {
-    return launch_lsm_mlp<float>(train, train_items, m, wgs, value, value_items, risk, s, paths_done, between);
+hipError_t launch_lsm_mlp_f32(const LsmArgs& train, uint32_t train_items, const MlpLsmArgs& m, const LsmArgs& judge,
+                              const LsmArgs& value, const LsmArgs& judge_value, uint32_t value_items, bool risk,
+                              hipStream_t s, const hipEvent_t* ev) {
+    return launch_lsm_mlp<float>(train, train_items, m, judge, value, judge_value, value_items, risk, s, ev);
 }
 
 hipError_t launch_lv_f32(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s) {

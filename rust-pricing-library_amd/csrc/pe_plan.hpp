@@ -300,10 +300,12 @@ hipError_t launch_lsm_f64(const LsmArgs& train, uint32_t train_items, const LsmA
 hipError_t launch_lsm_value_f32(const LsmArgs& value, uint32_t value_items, uint32_t basis, bool risk, hipStream_t s);
 hipError_t launch_lsm_f32(const LsmArgs& train, uint32_t train_items, const LsmArgs& value, uint32_t value_items,
                           uint32_t basis, bool risk, hipStream_t s, hipEvent_t paths_done, hipEvent_t between);
-hipError_t launch_lsm_mlp_f64(const LsmArgs& train, uint32_t train_items, const MlpLsmArgs& m, uint32_t wgs,
-                              const LsmArgs& value, uint32_t value_items, bool risk, hipStream_t s, hipEvent_t paths_done, hipEvent_t between);
-hipError_t launch_lsm_mlp_f32(const LsmArgs& train, uint32_t train_items, const MlpLsmArgs& m, uint32_t wgs,
-                              const LsmArgs& value, uint32_t value_items, bool risk, hipStream_t s, hipEvent_t paths_done, hipEvent_t between);
+hipError_t launch_lsm_mlp_f64(const LsmArgs& train, uint32_t train_items, const MlpLsmArgs& m, const LsmArgs& judge,
+                              const LsmArgs& value, const LsmArgs& judge_value, uint32_t value_items, bool risk,
+                              hipStream_t s, const hipEvent_t* ev);
+hipError_t launch_lsm_mlp_f32(const LsmArgs& train, uint32_t train_items, const MlpLsmArgs& m, const LsmArgs& judge,
+                              const LsmArgs& value, const LsmArgs& judge_value, uint32_t value_items, bool risk,
+                              hipStream_t s, const hipEvent_t* ev);
 hipError_t launch_mlp_gradient(const float* params, const float* x, const float* y, uint32_t rows, uint32_t rows_wg,
                                uint32_t wgs, float* partials, hipStream_t s);
 hipError_t launch_mlp_forward(const float* params, const float* x, uint32_t rows, float* out, hipStream_t s);

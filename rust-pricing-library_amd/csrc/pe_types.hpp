@@ -435,19 +435,17 @@ struct LsmDualArgs {
 // in-the-money training rows.
 struct MlpLsmArgs {
     LsmArgs lsm;
-    float* weights;            // FP32 master parameters
-    float* adam_m;
+    const float* init;         // initial parameters (every trade)
+    float* weights;            // FP32 master parameters [trade][stride]
+    float* adam_m;             // Adam moments [trade][stride]
     float* adam_v;
-    float* partials;           // [workgroup][parameters + 1]
-    float* rows;               // pooled training rows: (u1, u2, target) per row
-    double* stats;             // [workgroup][8]: count, sums of m, m^2, vol, vol^2, y, y^2
+    float4* rows;              // training rows (u1, u2, target, 0), one per training path of each trade
     LsmCuts* cuts;             // [trade][non-terminal date]
     unsigned int* error;       // bit 0: an exercise region needed more than kLsmMaxCuts cuts
-    uint32_t n_trades, n_dates, paths_per_trade;
-    uint32_t batch, epochs, rows_per_wg;
+    size_t cash_bytes;         // bytes of the training cash-flow vector (copied for the polynomial judge)
+    uint32_t n_trades, batch, epochs, pad;
     float lr, beta1, beta2, eps;
     float grid_lo, grid_hi;    // moneyness range of the exercise-region search
-    uint32_t call, pad;
 };
 
 // Kernel arguments of the local-vol Log-Euler kernel.

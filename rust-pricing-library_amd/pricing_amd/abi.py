@@ -380,6 +380,15 @@ class pe_lsm_report(C.Structure):
         ("abs_rank_tolerance", C.c_double),
         ("rel_rank_tolerance", C.c_double),
         ("max_matrix_elements", C.c_uint64),
+        ("network_price", pe_estimate),
+        ("judge_price", pe_estimate),
+        ("judge_in_sample_value", C.c_double),
+        ("fallback_used", C.c_int32),
+        ("pad", C.c_uint32),
+        ("network_training_ms", C.c_double),
+        ("judge_training_ms", C.c_double),
+        ("network_valuation_ms", C.c_double),
+        ("judge_valuation_ms", C.c_double),
     ]
 
 

@@ -140,7 +140,9 @@ class LsmReport:
     """Early-exercise diagnostics of one plan: in-sample value, valuation
     exercise counts and probabilities, the decision model per non-terminal
     date, dividend collisions, optional stopping indices, the engines'
-    domains, sizes and RQMC checksums, device ms (EarlyExerciseDiagnostics)."""
+    domains, sizes and RQMC checksums, device ms (EarlyExerciseDiagnostics);
+    for the network policy, its and the polynomial judge's out-of-sample
+    values and whether the judge's policy was kept (the fallback)."""
 
     def __init__(self, r, counts, decisions, probabilities, collisions, stops):
         n = r.exercise_dates
@@ -172,6 +174,15 @@ class LsmReport:
         self.abs_rank_tolerance = r.abs_rank_tolerance
         self.rel_rank_tolerance = r.rel_rank_tolerance
         self.max_matrix_elements = r.max_matrix_elements
+        # network policy: its own and the polynomial judge's out-of-sample values
+        self.network_price = abi.pe_estimate.from_buffer_copy(r.network_price)
+        self.judge_price = abi.pe_estimate.from_buffer_copy(r.judge_price)
+        self.judge_in_sample_value = r.judge_in_sample_value
+        self.fallback_used = bool(r.fallback_used)
+        self.network_training_ms = r.network_training_ms
+        self.judge_training_ms = r.judge_training_ms
+        self.network_valuation_ms = r.network_valuation_ms
+        self.judge_valuation_ms = r.judge_valuation_ms
 
 
 def evaluate_lsm(plans, backend="fp64", max_dates=1024, stopping_capacity=0):

@@ -222,13 +222,13 @@ def test_laguerre_batch_mixed_within_tenth_se():
                                                                              b.estimates[0].value, se)
 
 
-# ---- learned policy: one continuation network per date shared by the trades --
+# ---- learned policy: one continuation network per trade, judged by Laguerre --
 
-def _mlp_batch(method, train_log2, n=64, value_log2=13, dates=12, seed=21):
+def _mlp_batch(method, train_log2, n=64, value_log2=13, dates=12, seed=21, backend="mixed"):
     reqs = workloads.american_batch(n, seed=seed, train_log2=train_log2, value_log2=value_log2, dates=dates,
                                     method=method)
     plans = [engine.PricingPlan.compile(r) for r in reqs]
-    return engine.evaluate_lsm(plans, "mixed")
+    return engine.evaluate_lsm(plans, backend)
 
 
 def test_mlp_policy_is_deterministic_with_valid_regions():
@@ -238,25 +238,55 @@ def test_mlp_policy_is_deterministic_with_valid_regions():
         assert x.estimates[0].value == y.estimates[0].value
     for r, s in zip(rep_a, rep_b):
         assert r.exercise_counts == s.exercise_counts
+        assert r.network_price.value == s.network_price.value and r.judge_price.value == s.judge_price.value
         for c in r.decisions:
             assert c.region_cut_count <= 63
             cuts = list(c.region_cuts)[:c.region_cut_count]
             assert all(0.0 < x < 1.0 for x in cuts) and cuts == sorted(cuts)
 
 
-def test_mlp_policy_against_laguerre_policies():
-    """The network (one per date for the 64 trades, 2^9 training paths per
-    trade) stays within 3 standard errors on average of the per-trade
-    Laguerre regressions trained on 32 times more paths, valued on the same
-    paths (bars from the measured gaps: -1 SE against Laguerre on the same
-    paths, -2 SE against Laguerre on 2^14 paths)."""
+def test_judge_is_the_laguerre_policy_and_the_fallback_keeps_the_higher_value():
+    """In FP64 mode the judge is bit for bit the weighted-Laguerre plan on the
+    same training and valuation paths; the result is the higher of the
+    network's and the judge's out-of-sample values, and the exercise counts
+    are the kept policy's."""
+    nn, nrep = _mlp_batch(abi.PE_LSM_MLP, 10, n=16, backend="fp64")
+    lag, lrep = _mlp_batch(abi.PE_LSM_POLY_LAGUERRE, 10, n=16, backend="fp64")
+    for r, x, lr in zip(nrep, lag, lrep):
+        assert r.judge_price.value == x.estimates[0].value
+        assert r.judge_price.standard_error == x.estimates[0].standard_error
+        assert r.judge_in_sample_value == lr.in_sample_value
+    for res, r, lr in zip(nn, nrep, lrep):
+        keep = r.judge_price if r.fallback_used else r.network_price
+        assert res.estimates[0].value == keep.value == max(r.network_price.value, r.judge_price.value)
+        assert r.fallback_used == (r.judge_price.value > r.network_price.value)
+        if r.fallback_used:
+            assert r.exercise_counts == lr.exercise_counts
+        assert r.network_training_ms > 0.0 and r.judge_training_ms > 0.0
+
+
+def test_mlp_policy_within_one_standard_error_of_the_judge():
+    """64 trades, 2^14 training and 2^13 valuation paths each, 12 dates: the
+    per-trade networks price within 1 SE of the per-trade weighted-Laguerre
+    regressions on the same paths, trade by trade (measured at 2^14
+    valuation paths: mean +0.03 SE, worst -0.53 / +0.67 SE)."""
     import numpy as np
-    nn, _ = _mlp_batch(abi.PE_LSM_MLP, 9)
-    full, _ = _mlp_batch(abi.PE_LSM_POLY_LAGUERRE, 14)
-    se = np.array([r.estimates[0].standard_error for r in nn])
-    d = np.array([a.estimates[0].value - b.estimates[0].value for a, b in zip(nn, full)])
-    assert abs(np.mean(d / se)) < 3.0, np.mean(d / se)
-    assert np.max(np.abs(d) / se) < 8.0
+    _, rep = _mlp_batch(abi.PE_LSM_MLP, 14)
+    z = np.array([(r.network_price.value - r.judge_price.value) / r.judge_price.standard_error for r in rep])
+    print("network - judge in SE: mean %.3f min %.3f max %.3f" % (z.mean(), z.min(), z.max()), flush=True)
+    assert abs(z.mean()) < 0.5 and np.max(np.abs(z)) < 1.0, z
+
+
+def test_mlp_policy_gains_over_laguerre_on_few_training_paths():
+    """What the network gains: on 2^8 training paths per trade (50 dates) the
+    networks -- the European continuation as a control variate plus a trained
+    residual -- value the 64 trades above their own per-trade Laguerre
+    regressions on the same paths (measured: +2.3 SE on average)."""
+    import numpy as np
+    _, rep = _mlp_batch(abi.PE_LSM_MLP, 8, dates=50, value_log2=14)
+    z = np.array([(r.network_price.value - r.judge_price.value) / r.judge_price.standard_error for r in rep])
+    print("network - judge in SE at 2^8 training paths: mean %.3f" % z.mean(), flush=True)
+    assert z.mean() > 1.0, z.mean()
 
 
 def test_mlp_policy_captures_the_early_exercise_premium():
