@@ -523,6 +523,7 @@ typedef struct pe_lsm_report {
     double network_training_ms;     /* training paths + network induction (HIP events) */
     double judge_training_ms;       /* the polynomial judge's induction on the same paths */
     double network_valuation_ms, judge_valuation_ms;
+    uint64_t network_rows_trained;  /* rows run through the network: epochs x in-the-money rows, all dates */
 } pe_lsm_report;
 
 /* Prices `count` American plans by least-squares Monte Carlo

@@ -1381,7 +1381,8 @@ struct LsmLaunch {
     std::vector<BlockStat> host_judge_partials;
     std::vector<unsigned long long> host_judge_counts;
     std::vector<double> host_judge_in_sample;
-    DeviceBuffer mlp_init, weights, adam_m, adam_v, mlp_rows, cuts, mlp_err, judge_cash, judge_partials, judge_counts,
+    std::vector<unsigned long long> host_rows_trained;
+    DeviceBuffer rows_trained, mlp_init, weights, adam_m, adam_v, mlp_rows, cuts, mlp_err, judge_cash, judge_partials, judge_counts,
         judge_in_sample;
     ~LsmLaunch() {
         for (hipEvent_t e : ev)
@@ -1452,6 +1453,8 @@ void prepare_mlp(LsmLaunch& L, std::vector<HostPlan*>& plans, size_t cash_bytes)
     L.host_judge_partials.resize(L.n_partials);
     L.host_judge_counts.resize(L.n_counts);
     L.host_judge_in_sample.resize(n);
+    L.rows_trained.reserve(n * sizeof(unsigned long long));
+    L.host_rows_trained.resize(n);
     MlpLsmArgs& m = L.mlp_args;
     m.init = L.mlp_init.as<float>();
     m.weights = L.weights.as<float>();
@@ -1460,6 +1463,7 @@ void prepare_mlp(LsmLaunch& L, std::vector<HostPlan*>& plans, size_t cash_bytes)
     m.rows = L.mlp_rows.as<float4>();
     m.cuts = L.cuts.as<LsmCuts>();
     m.error = L.mlp_err.as<unsigned int>();
+    m.rows_trained = L.rows_trained.as<unsigned long long>();
     m.cash_bytes = cash_bytes;
     m.n_trades = static_cast<uint32_t>(n);
     m.batch = B;
@@ -1777,6 +1781,9 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
         check(copy_on_stream(L.host_judge_in_sample.data(), L.judge_in_sample.ptr, plans.size() * sizeof(double),
                              hipMemcpyDeviceToHost),
               "judge in-sample copy");
+        check(copy_on_stream(L.host_rows_trained.data(), L.rows_trained.ptr, plans.size() * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost),
+              "rows-trained copy");
     }
     for (size_t ti = 0; ti < plans.size(); ++ti) {
         const HostPlan& p = *plans[ti];
@@ -1849,6 +1856,7 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
         r.judge_in_sample_value = 0.0;
         r.fallback_used = 0;
         r.network_training_ms = r.judge_training_ms = r.network_valuation_ms = r.judge_valuation_ms = 0.0;
+        r.network_rows_trained = 0;
         if (L.mlp) {
             r.network_price = network;
             r.judge_price = judge.estimates[0];
@@ -1858,6 +1866,7 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
             r.judge_training_ms = judge_train_ms;
             r.network_valuation_ms = nn_value_ms;
             r.judge_valuation_ms = judge_value_ms;
+            r.network_rows_trained = L.host_rows_trained[ti];
         }
         if (r.stopping_indices && r.stopping_capacity) {
             const uint64_t k = std::min<uint64_t>(r.stopping_capacity, vtraj);

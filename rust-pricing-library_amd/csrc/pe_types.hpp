@@ -442,6 +442,7 @@ struct MlpLsmArgs {
     float4* rows;              // training rows (u1, u2, target, 0), one per training path of each trade
     LsmCuts* cuts;             // [trade][non-terminal date]
     unsigned int* error;       // bit 0: an exercise region needed more than kLsmMaxCuts cuts
+    unsigned long long* rows_trained;   // [trade]: rows run through the network (epochs x rows, all dates)
     size_t cash_bytes;         // bytes of the training cash-flow vector (copied for the polynomial judge)
     uint32_t n_trades, batch, epochs, pad;
     float lr, beta1, beta2, eps;

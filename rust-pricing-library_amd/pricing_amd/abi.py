@@ -389,6 +389,7 @@ class pe_lsm_report(C.Structure):
         ("judge_training_ms", C.c_double),
         ("network_valuation_ms", C.c_double),
         ("judge_valuation_ms", C.c_double),
+        ("network_rows_trained", C.c_uint64),
     ]
 
 

@@ -183,6 +183,7 @@ class LsmReport:
         self.judge_training_ms = r.judge_training_ms
         self.network_valuation_ms = r.network_valuation_ms
         self.judge_valuation_ms = r.judge_valuation_ms
+        self.network_rows_trained = r.network_rows_trained
 
 
 def evaluate_lsm(plans, backend="fp64", max_dates=1024, stopping_capacity=0):
