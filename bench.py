@@ -11,13 +11,18 @@ Workload (one "step" = one evaluation of the whole job):
   products: up-and-out call K=100, B=130 with daily monitoring and an
   arithmetic Asian call K=100, priced on the same paths in one launch.
 
-A second block ("batch_bs") times the batched BS/Black-76 workload: 4096
-uneven trades with affine dividends and Delta/Vega, mixed vs FP64 device modes
-and the CPU oracle on a sample of the trades.  A third ("rough_smile") times
-the lifted Heston 21 x 8 European smile on one 2^20-path RQMC set, and
-("rough_bermudan") a Bermudan put on the same model by least squares on
-(spot, variance).  A fourth ("lsm_american") prices 256 American puts by LSM with the weighted-Laguerre
-policy and with the learned (MLP) policy.
+Further blocks, each with its own workload and roofline:
+  * "european_bs": the single European call in the FP64 mode (price, Delta,
+    Vega against the closed form);
+  * "batch_bs": 4096 uneven BS/Black-76 trades with affine dividends and
+    Delta/Vega, mixed vs FP64 device modes and the CPU oracle on a sample;
+  * "rough_smile": the lifted Heston 21 x 8 European smile on one
+    2^20-path RQMC set;
+  * "rough_bermudan": a Bermudan put on the same model by least squares on
+    (spot, variance);
+  * "graph_payoff": product graphs through the device tape interpreter;
+  * "lsm_american": 256 American puts by LSM with the weighted-Laguerre
+    policy and with the learned (MLP) policy.
 
 Prints one JSON line.  --gpus above 1 is refused: this build runs one GPU.
 """
@@ -87,14 +92,14 @@ def time_mode(plans, backend, steps, warmup):
     return res, wall, kernel_ms
 
 
-def cpu_baseline(reqs, target_seconds):
-    """The CPU FP64 oracle on all host cores on a bounded sample of the same
-    workload: the same plans with fewer RQMC points and scrambles, grown until
-    the sample takes about `target_seconds`."""
+def cpu_baseline(reqs, target_seconds, cores=None):
+    """The CPU FP64 oracle on all host cores (or `cores` threads) on a bounded
+    sample of the same workload: the same plans with fewer RQMC points and
+    scrambles, grown until the sample takes about `target_seconds`."""
     from _support import oracle
 
     lib = oracle()
-    cores = int(lib.oracle_hardware_threads())
+    cores = int(lib.oracle_hardware_threads()) if cores is None else cores
     points, scrambles = 1 << 10, 2
     while True:
         sample = []
@@ -127,6 +132,77 @@ def cpu_baseline(reqs, target_seconds):
                       f"barrier + Asian on shared paths, FP64 oracle, {dt:.2f} s wall"}
 
 
+def european_block(no_cpu):
+    """Config 1: European call under Black-Scholes in the FP64 device mode
+    (S0 = K = 100, r = 5%, sigma = 20%, T = 1, one exact lognormal step,
+    2^16 pseudo paths from seed 0x0123456789abcdef, pathwise Delta and Vega,
+    block 4096), checked against the closed form.  At 2^16 paths one launch
+    is latency-bound (a few microseconds of work), so the same plan is also
+    timed at 2^24 paths for the FP64 kernel's roofline; work model: 120 FP64
+    flops per path (normal draw, exact step, payoff, pathwise Delta/Vega and
+    the three-component reductions)."""
+    import math
+
+    def closed_form():
+        s, k, r, v, t = 100.0, 100.0, 0.05, 0.2, 1.0
+        d1 = (math.log(s / k) + (r + 0.5 * v * v) * t) / (v * math.sqrt(t))
+        d2 = d1 - v * math.sqrt(t)
+        n = lambda x: 0.5 * math.erfc(-x / math.sqrt(2.0))
+        return (s * n(d1) - k * math.exp(-r * t) * n(d2), n(d1),
+                s * math.exp(-0.5 * d1 * d1) / math.sqrt(2.0 * math.pi) * math.sqrt(t))
+
+    def timed(units, reps):
+        plan = engine.PricingPlan.compile(workloads.european_bs(units=units),
+                                          request.exec_policy(1, 4096))
+        plan.evaluate("fp64")
+        walls, kms = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            res = plan.evaluate("fp64")
+            walls.append(time.perf_counter() - t0)
+            kms.append(engine.last_kernel_ms()[0])
+        return res, min(walls), sum(kms) / len(kms)
+
+    price, delta, vega = closed_form()
+    res, wall, kms = timed(1 << 16, 20)
+    est = res.estimates
+    block = {
+        "workload": "European call, BS, S0=K=100, r=5%, q=0, sigma=20%, T=1, one exact step, 2^16 pseudo paths, "
+                    "pathwise Delta/Vega, FP64 device mode",
+        "paths": res.evaluated_paths,
+        "wall_ms": 1e3 * wall, "kernel_ms": kms,
+        "value": res.evaluated_paths / wall, "unit": "paths/s (one path-step each)",
+        "price": {"mc": est[0].value, "se": est[0].standard_error, "closed_form": price,
+                  "error_in_se": (est[0].value - price) / est[0].standard_error},
+        "delta": {"mc": est[1].value, "se": est[1].standard_error, "closed_form": delta,
+                  "error_in_se": (est[1].value - delta) / est[1].standard_error},
+        "vega": {"mc": est[2].value, "se": est[2].standard_error, "closed_form": vega,
+                 "error_in_se": (est[2].value - vega) / est[2].standard_error},
+    }
+    big, wall24, kms24 = timed(1 << 24, 5)
+    flops = 120.0 * big.evaluated_paths
+    block["throughput_2p24"] = {"paths": big.evaluated_paths, "wall_ms": 1e3 * wall24, "kernel_ms": kms24,
+                                "value": big.evaluated_paths / wall24}
+    block["roofline"] = {"bound": "valu", "kernel": "bs_batch_kernel<double, risk> at 2^24 paths",
+                         "achieved": flops / (kms24 * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": flops / (kms24 * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, "flops_per_path": 120,
+                         "traffic": None}
+    if not no_cpu:
+        from _support import oracle, oracle_evaluate
+
+        cores = int(oracle().oracle_hardware_threads())
+        req = workloads.european_bs(units=1 << 22)
+        for n in (1, cores):
+            t0 = time.perf_counter()
+            r = oracle_evaluate(req, n, 4096)
+            dt = time.perf_counter() - t0
+            block["cpu_baseline" if n == cores else "cpu_baseline_one_core"] = {
+                "value": r.evaluated_paths / dt, "unit": "paths/s", "cores": n, "kind": "port",
+                "sample": f"the same plan at 2^22 paths, FP64 oracle, {dt:.2f} s wall"}
+        block["speedup_vs_cpu_all_cores"] = block["throughput_2p24"]["value"] / block["cpu_baseline"]["value"]
+    return block
+
+
 def batch_block(steps, cpu_seconds, no_cpu, no_fp64):
     """Config 3: 4096 uneven BS/Black-76 trades with affine dividends, Delta and
     Vega, one launch per kernel flavour; mixed vs FP64 device modes and the CPU
@@ -150,6 +226,12 @@ def batch_block(steps, cpu_seconds, no_cpu, no_fp64):
         "unit": "path-steps/s",
         "ms_per_step": 1e3 * wall,
         "kernel_ms": kernel_ms,
+        # work model: normal draw 34 + BS step 8 + payoff/monitor 2-14 +
+        # pathwise adjoint 8 = 64 flops per path-step for the risk trades
+        "roofline": {"bound": "valu", "kernel": "bs_batch_kernel<float, *> (all flavours of the launch)",
+                     "achieved": path_steps * 64 / (kernel_ms * 1e-3) / 1e12, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": path_steps * 64 / (kernel_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+                     "flops_per_path_step": 64, "traffic": None},
     }
     if not no_fp64:
         engine.evaluate_batch(plans, "fp64")
@@ -245,6 +327,14 @@ def rough_block(steps, cpu_seconds, no_cpu, no_fp64):
         "kernel_ms": engine.last_kernel_ms()[0],
         "atm_1y": {"price": mixed[4 * 21 + 10].estimates[0].value, "se": mixed[4 * 21 + 10].estimates[0].standard_error},
     }
+    # work model: 168 flops per path-step (20-factor step with 2 normals +
+    # payoff); the mixed scheme runs the variance process in FP64, so the
+    # FP64 vector peak is the bound it is measured against
+    kms = block["kernel_ms"]
+    block["roofline"] = {"bound": "valu", "kernel": "rough_path_kernel<float, 20> + rough_payoff_kernel",
+                         "achieved": path_steps * 168 / (kms * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": path_steps * 168 / (kms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                         "flops_per_path_step": 168, "traffic": None}
     if not no_fp64:
         engine.evaluate_shared(plans, "fp64")
         t0 = time.perf_counter()
@@ -511,6 +601,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-european", action="store_true", help="skip the single European (FP64) block")
     ap.add_argument("--no-batch", action="store_true", help="skip the 4096-trade BS/Black-76 batch block")
     ap.add_argument("--no-rough", action="store_true", help="skip the lifted Heston smile block")
     ap.add_argument("--no-graph", action="store_true", help="skip the graph-product interpreter block")
@@ -554,8 +645,8 @@ def main():
             "paths": res[0].evaluated_paths,
             "time_steps": res[0].time_steps,
             "path_steps_per_step": path_steps,
-            "precision": "mixed: FP32 inverse normal, exp/sqrt, variance lookup; FP64 log-state, bridge, "
-                         "payoff accumulators and statistics",
+            "precision": "mixed: FP32 inverse normal, exp/sqrt, variance lookup and Brownian bridge; FP64 "
+                         "log-state, payoff accumulators and statistics",
         },
         "roofline": {
             "bound": "valu",
@@ -564,6 +655,8 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
             "traffic": measured_traffic(),
+            "traffic_source": "committed calibration: rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel scaled by "
+                              "the same counters on a copy of known size (profiles/lv_kernel_traffic.json)",
             "kernel": "lv_kernel<float>",
             "avg_kernel_ms": avg_kernel_ms,
             "flops_per_path_step": FLOPS_PER_PATH_STEP,
@@ -594,8 +687,13 @@ def main():
         cb = cpu_baseline(reqs, args.cpu_seconds)
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu_all_cores"] = value / cb["value"]
+        one = cpu_baseline(reqs, 0.5 * args.cpu_seconds, cores=1)
+        out["cpu_baseline_one_core"] = one
+        out["speedup_vs_cpu_one_core"] = value / one["value"]
     else:
         out["cpu_baseline"] = None
+    if not args.no_european:
+        out["european_bs"] = european_block(args.no_cpu)
     if not args.no_batch:
         out["batch_bs"] = batch_block(3, args.cpu_seconds, args.no_cpu, args.no_fp64)
     if not args.no_rough:
