@@ -377,7 +377,8 @@ def rough_bermudan_block(cpu_seconds, no_cpu, no_fp64):
     (spot, variance) with standardized monomials of total degree 2 and CPQR
     rules; 2^18 training + 2^18 independent valuation pseudo paths.  The
     policy trains on FP64 paths in both modes; the mixed mode values on the
-    mixed scheme's paths."""
+    mixed scheme's paths.  The "network" leg prices the same Bermudan with the
+    network continuation on (spot, variance)."""
     model = workloads.lifted_heston_model(engine.lifted_from_rough(0.1, 20, 2.5))
     req = workloads.rough_bermudan(units=1 << 18, train_units=1 << 18)
     plan = engine.RoughPlan.compile(req, model, 2.0 / 256)
@@ -419,6 +420,31 @@ def rough_bermudan_block(cpu_seconds, no_cpu, no_fp64):
         block["mixed_over_fp64"] = fms / ms
         block["mixed_minus_fp64_in_se"] = (mixed.estimates[0].value - f64.estimates[0].value) / \
             f64.estimates[0].standard_error
+    # the network continuation on (spot, variance), judged by the polynomial
+    # policy on the same paths (one launch prices both)
+    nreq = workloads.rough_bermudan(units=1 << 18, train_units=1 << 18, method=abi.PE_LSM_MLP)
+    nplan = engine.RoughPlan.compile(nreq, model, 2.0 / 256)
+    engine.evaluate_lsm([nplan], "mixed")
+    t0 = time.perf_counter()
+    (nres,), (nrep,) = engine.evaluate_lsm([nplan], "mixed")
+    nwall = time.perf_counter() - t0
+    nn = {
+        "config": "one 2-64-64-1 SiLU network per exercise date on (S/K, V), BF16 MFMA, Adam lr 1e-3, 4 epochs, "
+                  "batch 8192, 2^18 training paths (in-the-money rows), control variate: Black value with the "
+                  "expected average variance; the polynomial policy above is its judge on the same paths",
+        "price": nrep.network_price.value, "se": nrep.network_price.standard_error,
+        "minus_polynomial_in_se": (nrep.network_price.value - nrep.judge_price.value) / nrep.judge_price.standard_error,
+        "network_training_ms": nrep.network_training_ms, "network_valuation_ms": nrep.network_valuation_ms,
+        "time_over_polynomial": (nrep.network_training_ms + nrep.network_valuation_ms) / ms,
+        "fallback_used": bool(nrep.fallback_used), "kept_price": nres.estimates[0].value,
+        "rows_trained": int(nrep.network_rows_trained), "wall_ms": 1e3 * nwall,
+    }
+    if not no_fp64:
+        (_,), (nf64,) = engine.evaluate_lsm([nplan], "fp64")
+        nn["fp64_price"] = nf64.network_price.value
+        nn["mixed_minus_fp64_in_se"] = (nrep.network_price.value - nf64.network_price.value) / \
+            nf64.network_price.standard_error
+    block["network"] = nn
     if not no_cpu:
         from _support import oracle_rough_lsm
 

@@ -649,8 +649,20 @@ struct RoughLsmLaunch {
     std::vector<RoughLsmDecision> host_decisions;
     std::vector<unsigned long long> host_counts;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};   // start, paths done, training done, valuation done
+    // network policy (PE_LSM_MLP): the polynomial policy above is its judge
+    bool nn = false;
+    RoughNnArgs nn_args{};
+    DeviceBuffer d_times, nn_cash, nn_init, nn_weights, nn_adam_m, nn_adam_v, nn_rows, nn_grad, nn_stats, nn_offsets,
+        nn_norms, nn_in_sample, nn_partials, nn_counts;
+    std::vector<BlockStat> host_nn_partials;
+    std::vector<unsigned long long> host_nn_counts;
+    std::vector<RoughNnNorm> host_norms;
+    unsigned long long rows_trained = 0;
+    hipEvent_t nn_ev[3] = {nullptr, nullptr, nullptr};   // network start, trained, valued
     ~RoughLsmLaunch() {
         for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t e : nn_ev)
             if (e) (void)hipEventDestroy(e);
     }
 };
@@ -858,6 +870,97 @@ void prepare_rough(std::vector<HostPlan*>& plans, pe_backend backend) {
     p0.rough_launch.reset(L.release());
 }
 
+// The network policy of a Bermudan under lifted Heston (pe_rough_nn.inc):
+// one network per non-terminal date, trained on the polynomial policy's FP64
+// training stores with its own copy of the cash flows; 64 gradient
+// workgroups of 128 rows per Adam step (batch a multiple of 128).
+void prepare_rough_nn(const HostPlan& p, RoughLsmLaunch& L, uint32_t value_items) {
+    const pe_lsm& l = p.req.lsm;
+    if (l.mlp_hidden != 64) throw PlanError(PE_ERR_UNSUPPORTED, "MLP hidden width must be 64 on the device");
+    if (l.mlp_epochs == 0 || !(l.mlp_learning_rate > 0.0)) throw PlanError(PE_ERR_INVALID_ARGUMENT, "MLP epochs / rate");
+    const uint32_t B = l.mlp_batch;
+    if (B == 0 || B % 128 != 0 || B > (1u << 20))
+        throw PlanError(PE_ERR_INVALID_ARGUMENT, "MLP batch must be a positive multiple of 128, at most 2^20");
+    const uint32_t G = std::max<uint32_t>(1, std::min<uint32_t>(64, B / 128));
+    if (B % (128 * G) != 0) throw PlanError(PE_ERR_INVALID_ARGUMENT, "MLP batch must split into 128-row tiles");
+    const RoughLsmArgs& a = L.args;
+    const uint32_t dates = a.n_dates;
+    const size_t stride = (kMlpParameterCount + 3) / 4 * 4;
+    L.nn = true;
+    std::vector<double> times(p.obs_times.begin(), p.obs_times.end());
+    L.d_times.upload(times);
+    std::vector<float> init(kMlpParameterCount, 0.0f);
+    uint64_t state = l.mlp_seed;
+    auto uniform = [&]() {
+        state += 0x9E3779B97F4A7C15ull;
+        uint64_t z = state;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return 2.0 * (static_cast<double>(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+    };
+    for (int k = 0; k < 128; ++k) init[k] = static_cast<float>(uniform() * std::sqrt(6.0 / 2.0));
+    for (int k = 192; k < 4288; ++k) init[k] = static_cast<float>(uniform() * std::sqrt(6.0 / 64.0));
+    L.nn_init.upload(init);   // output layer at zero: the untrained network is the control variate
+    const size_t nets = std::max<uint32_t>(1, dates - 1);
+    L.nn_cash.reserve(a.train_paths * sizeof(double));
+    L.nn_weights.reserve(nets * stride * sizeof(float));
+    L.nn_adam_m.reserve(stride * sizeof(float));
+    L.nn_adam_v.reserve(stride * sizeof(float));
+    L.nn_rows.reserve(std::max<uint64_t>(1, a.train_paths) * 4 * sizeof(float));
+    L.nn_grad.reserve(static_cast<size_t>(G) * (kMlpParameterCount + 1) * sizeof(float));
+    L.nn_stats.reserve(static_cast<size_t>(a.n_wg) * 8 * sizeof(double));
+    L.nn_offsets.reserve(static_cast<size_t>(a.n_wg) * sizeof(uint32_t));
+    L.nn_norms.reserve(nets * sizeof(RoughNnNorm));
+    L.nn_in_sample.reserve(sizeof(double));
+    L.nn_partials.reserve(static_cast<size_t>(value_items) * sizeof(BlockStat));
+    L.nn_counts.reserve(dates * sizeof(unsigned long long));
+    L.host_nn_partials.resize(value_items);
+    L.host_nn_counts.resize(dates);
+    L.host_norms.resize(nets);
+    RoughNnArgs& A = L.nn_args;
+    A.ex_nodes = a.ex_nodes;
+    A.ex_discount = a.ex_discount;
+    A.ex_scale = a.ex_scale;
+    A.ex_times = L.d_times.as<double>();
+    A.store_s = a.store_s;
+    A.store_v = a.store_v;
+    A.cash = L.nn_cash.as<double>();
+    A.init = L.nn_init.as<float>();
+    A.weights = L.nn_weights.as<float>();
+    A.adam_m = L.nn_adam_m.as<float>();
+    A.adam_v = L.nn_adam_v.as<float>();
+    A.rows = L.nn_rows.as<float4>();
+    A.grad = L.nn_grad.as<float>();
+    A.stats = L.nn_stats.as<double>();
+    A.offsets = L.nn_offsets.as<uint32_t>();
+    A.norms = L.nn_norms.as<RoughNnNorm>();
+    A.in_sample = L.nn_in_sample.as<double>();
+    A.value_partials = L.nn_partials.ptr;
+    A.counts = L.nn_counts.as<unsigned long long>();
+    A.error = a.error;
+    A.train_paths = a.train_paths;
+    A.value_chunk = a.value_chunk;
+    A.strike = a.strike;
+    A.notional = a.notional;
+    A.itm_tol = a.itm_tol;
+    A.kappa = p.lifted.kappa;
+    A.theta = p.lifted.theta;
+    A.lr = static_cast<float>(l.mlp_learning_rate);
+    A.beta1 = 0.9f;
+    A.beta2 = 0.999f;
+    A.eps = 1e-8f;
+    A.n_dates = dates;
+    A.call = a.call;
+    A.n_wg = a.n_wg;
+    A.batch = B;
+    A.epochs = l.mlp_epochs;
+    A.grad_wgs = G;
+    A.rows_per_wg = B / G;
+    A.value_items = value_items;
+    for (hipEvent_t& e : L.nn_ev) check(hipEventCreate(&e), "hipEventCreate");
+}
+
 // A Bermudan under lifted Heston: the training paths, the backward
 // induction over the exercise dates and the valuation (pe_rough_lsm.inc).
 void prepare_rough_lsm(HostPlan& p, pe_backend backend) {
@@ -954,6 +1057,7 @@ void prepare_rough_lsm(HostPlan& p, pe_backend backend) {
     L->host_decisions.resize(dates);
     L->host_counts.resize(dates);
     for (hipEvent_t& e : L->ev) check(hipEventCreate(&e), "hipEventCreate");
+    if (p.req.lsm.method == PE_LSM_MLP) prepare_rough_nn(p, *L, chunks * value_scr);
     p.rough_lsm_launch = std::move(L);
 }
 
@@ -973,16 +1077,60 @@ void evaluate_rough_lsm(HostPlan& p, pe_backend backend, pe_result* out, pe_lsm_
     check(hipMemcpyAsync(L.counts.ptr, zeros.data(), L.dates * sizeof(unsigned long long), hipMemcpyHostToDevice,
                          t_stream),
           "counts");
+    uint32_t* nn_stops = nullptr;   // the network policy's stopping indices are the ones reported
+    if (L.nn) {
+        nn_stops = a.stops;
+        a.stops = nullptr;
+    }
     check(hipEventRecord(L.ev[0], t_stream), "hipEventRecord");
-    check(backend == PE_BACKEND_GPU_FP64 ? launch_rough_lsm_f64(a, L.train_grid, L.value_grid, t_stream, L.ev[1], L.ev[2])
-                                         : launch_rough_lsm_f32(a, L.train_grid, L.value_grid, t_stream, L.ev[1], L.ev[2]),
+    void* nn_cash = L.nn ? L.nn_cash.ptr : nullptr;
+    check(backend == PE_BACKEND_GPU_FP64
+              ? launch_rough_lsm_f64(a, L.train_grid, L.value_grid, t_stream, L.ev[1], L.ev[2], nn_cash)
+              : launch_rough_lsm_f32(a, L.train_grid, L.value_grid, t_stream, L.ev[1], L.ev[2], nn_cash),
           "rough LSM launch");
     check(hipEventRecord(L.ev[3], t_stream), "hipEventRecord");
-    check(hipEventSynchronize(L.ev[3]), "hipEventSynchronize");
-    float path_ms = 0.0f, train_ms = 0.0f, value_ms = 0.0f;
+    if (L.nn) {
+        // the network policy: per date a sweep + reduction, the row count
+        // read back, then compaction and the Adam steps; then date 0's
+        // in-sample pass and the valuation
+        RoughNnArgs A = L.nn_args;
+        A.stops = nn_stops;
+        check(hipMemsetAsync(L.nn_counts.ptr, 0, L.dates * sizeof(unsigned long long), t_stream), "counts");
+        check(hipEventRecord(L.nn_ev[0], t_stream), "hipEventRecord");
+        uint32_t adam_t = 0;
+        L.rows_trained = 0;
+        check(hipMemsetAsync(L.nn_adam_m.ptr, 0, L.nn_adam_m.bytes, t_stream), "Adam state");
+        check(hipMemsetAsync(L.nn_adam_v.ptr, 0, L.nn_adam_v.bytes, t_stream), "Adam state");
+        for (int d = static_cast<int>(L.dates) - 2; d >= 0; --d) {
+            const uint32_t dd = static_cast<uint32_t>(d);
+            check(launch_rough_nn_date(A, dd, t_stream), "network sweep");
+            RoughNnNorm nm{};
+            check(copy_on_stream(&nm, L.nn_norms.as<RoughNnNorm>() + dd, sizeof(nm), hipMemcpyDeviceToHost),
+                  "row count copy");
+            if (nm.n == 0) continue;
+            const uint32_t steps = A.epochs * ((nm.n + A.batch - 1) / A.batch);
+            check(launch_rough_nn_train(A, dd, steps, adam_t, t_stream), "network training");
+            adam_t += steps;
+            L.rows_trained += static_cast<unsigned long long>(A.epochs) * nm.n;
+        }
+        check(launch_rough_nn_final(A, t_stream), "network in-sample pass");
+        check(hipEventRecord(L.nn_ev[1], t_stream), "hipEventRecord");
+        check(backend == PE_BACKEND_GPU_FP64 ? launch_rough_nn_value_f64(A, a.value, L.value_grid, t_stream)
+                                             : launch_rough_nn_value_f32(A, a.value, L.value_grid, t_stream),
+              "network valuation");
+        check(hipEventRecord(L.nn_ev[2], t_stream), "hipEventRecord");
+        check(hipEventSynchronize(L.nn_ev[2]), "hipEventSynchronize");
+    } else {
+        check(hipEventSynchronize(L.ev[3]), "hipEventSynchronize");
+    }
+    float path_ms = 0.0f, train_ms = 0.0f, value_ms = 0.0f, nn_train_ms = 0.0f, nn_value_ms = 0.0f;
     check(hipEventElapsedTime(&path_ms, L.ev[0], L.ev[1]), "hipEventElapsedTime");
     check(hipEventElapsedTime(&train_ms, L.ev[0], L.ev[2]), "hipEventElapsedTime");
     check(hipEventElapsedTime(&value_ms, L.ev[2], L.ev[3]), "hipEventElapsedTime");
+    if (L.nn) {
+        check(hipEventElapsedTime(&nn_train_ms, L.nn_ev[0], L.nn_ev[1]), "hipEventElapsedTime");
+        check(hipEventElapsedTime(&nn_value_ms, L.nn_ev[1], L.nn_ev[2]), "hipEventElapsedTime");
+    }
     unsigned long long err = none;
     check(copy_on_stream(&err, L.err.ptr, sizeof(err), hipMemcpyDeviceToHost), "error copy");
     if (err != none) {
@@ -1003,18 +1151,56 @@ void evaluate_rough_lsm(HostPlan& p, pe_backend backend, pe_result* out, pe_lsm_
     check(copy_on_stream(&in_sample, L.in_sample.ptr, sizeof(double), hipMemcpyDeviceToHost), "in-sample copy");
     std::memset(out, 0, sizeof(*out));
     assemble(p, L.host_partials, L.chunks, L.scrambles, 1, 0, 1, out);
+    // network policy: its out-of-sample value against the polynomial judge's
+    // on the same valuation paths; the higher one is the price
+    pe_result judge = *out;
+    bool fallback = false;
+    double nn_in_sample = 0.0;
+    pe_estimate network_est{};
+    if (L.nn) {
+        check(copy_on_stream(L.host_nn_partials.data(), L.nn_partials.ptr, L.host_nn_partials.size() * sizeof(BlockStat),
+                             hipMemcpyDeviceToHost),
+              "network partials copy");
+        check(copy_on_stream(L.host_nn_counts.data(), L.nn_counts.ptr, L.dates * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost),
+              "network counts copy");
+        check(copy_on_stream(&nn_in_sample, L.nn_in_sample.ptr, sizeof(double), hipMemcpyDeviceToHost),
+              "network in-sample copy");
+        pe_result network{};
+        assemble(p, L.host_nn_partials, L.chunks, L.scrambles, 1, 0, 1, &network);
+        network_est = network.estimates[0];
+        fallback = judge.estimates[0].value > network.estimates[0].value;
+        if (!fallback) *out = network;
+    }
+    const std::vector<unsigned long long>& counts = (L.nn && !fallback) ? L.host_nn_counts : L.host_counts;
     out->time_steps = L.n;
     out->path_steps = out->evaluated_paths * L.n;
     out->sobol_dimension = a.value.rqmc ? 2u * L.n : 0u;
-    out->device_ms = train_ms + value_ms;
-    g_kernel_ms = train_ms + value_ms;
+    out->device_ms = train_ms + value_ms + nn_train_ms + nn_value_ms;
+    g_kernel_ms = out->device_ms;
     g_kernel_launches = 3 + 6 * (L.dates > 1 ? L.dates - 1 : 0);
     if (!report) return;
     pe_lsm_report& r = *report;
     const pe_engine& te = p.req.lsm.training_engine;
-    r.in_sample_value = in_sample;
-    r.training_ms = train_ms;
-    r.valuation_ms = value_ms;
+    r.in_sample_value = L.nn ? nn_in_sample : in_sample;
+    r.training_ms = train_ms + nn_train_ms;
+    r.valuation_ms = value_ms + nn_value_ms;
+    r.network_price = r.judge_price = pe_estimate{};
+    r.judge_in_sample_value = 0.0;
+    r.fallback_used = 0;
+    r.network_training_ms = r.judge_training_ms = r.network_valuation_ms = r.judge_valuation_ms = 0.0;
+    r.network_rows_trained = 0;
+    if (L.nn) {
+        r.network_price = network_est;
+        r.judge_price = judge.estimates[0];
+        r.judge_in_sample_value = in_sample;
+        r.fallback_used = fallback ? 1 : 0;
+        r.network_training_ms = path_ms + nn_train_ms;
+        r.judge_training_ms = train_ms - path_ms;
+        r.network_valuation_ms = nn_value_ms;
+        r.judge_valuation_ms = value_ms;
+        r.network_rows_trained = L.rows_trained;
+    }
     r.path_ms = path_ms;
     r.exercise_dates = L.dates;
     r.training_random_domain = te.kind == PE_ENGINE_RQMC ? PE_DOMAIN_RQMC_SCRAMBLE : PE_DOMAIN_LSM_TRAIN;
@@ -1041,9 +1227,9 @@ void evaluate_rough_lsm(HostPlan& p, pe_backend backend, pe_result* out, pe_lsm_
               "stopping indices copy");
     }
     for (uint32_t d = 0; d < L.dates && d < r.capacity; ++d) {
-        if (r.exercise_counts) r.exercise_counts[d] = L.host_counts[d];
+        if (r.exercise_counts) r.exercise_counts[d] = counts[d];
         if (r.exercise_probabilities)
-            r.exercise_probabilities[d] = static_cast<double>(L.host_counts[d]) / static_cast<double>(vtraj);
+            r.exercise_probabilities[d] = static_cast<double>(counts[d]) / static_cast<double>(vtraj);
         if (r.dividend_collisions) r.dividend_collisions[d] = 0;
         if (!r.decisions || d + 1 >= L.dates) continue;
         const RoughLsmDecision& s = L.host_decisions[d];

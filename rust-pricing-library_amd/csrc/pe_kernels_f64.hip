@@ -9,6 +9,7 @@
 #include "pe_lsm_dual.inc"
 #include "pe_mlp.inc"
 #include "pe_mlp_lsm.inc"
+#include "pe_rough_nn.inc"
 #include "pe_plan.hpp"
 
 namespace pe {
@@ -22,8 +23,12 @@ hipError_t launch_bs_batch_f64(const BsBatchArgs& a, bool risk, bool cont, uint3
 }
 
 hipError_t launch_rough_lsm_f64(const RoughLsmArgs& a, dim3 train_grid, dim3 value_grid, hipStream_t s,
-                               hipEvent_t paths_done, hipEvent_t between) {
-    return launch_rough_lsm<double>(a, train_grid, value_grid, s, paths_done, between);
+                               hipEvent_t paths_done, hipEvent_t between, void* cash_copy) {
+    return launch_rough_lsm<double>(a, train_grid, value_grid, s, paths_done, between, cash_copy);
+}
+
+hipError_t launch_rough_nn_value_f64(const RoughNnArgs& A, const RoughArgs& a, dim3 grid, hipStream_t s) {
+    return launch_rough_nn_value<double>(A, a, grid, s);
 }
 
 hipError_t launch_rough_f64(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 grid, uint32_t payoff_items,

@@ -538,7 +538,8 @@ void compile_rough_plan(HostPlan& p, const pe_request& carrier, const pe_lifted_
         unsupported("lifted Heston on the device prices European, Digital and Bermudan (American) payoffs");
     if (american) {
         const pe_lsm& l = carrier.lsm;
-        if (l.method != PE_LSM_POLY_CPQR) unsupported("lifted Heston early exercise: CPQR monomial policy only");
+        if (l.method != PE_LSM_POLY_CPQR && l.method != PE_LSM_MLP)
+            unsupported("lifted Heston early exercise: the CPQR monomial or the network policy");
         if (l.max_degree < 1 || l.max_degree > 2) unsupported("lifted Heston early exercise: total degree 1 or 2");
         if (carrier.engine.brownian_bridge || l.training_engine.brownian_bridge)
             invalid("no Brownian bridge for lifted Heston early exercise");

@@ -288,9 +288,14 @@ hipError_t launch_rough_f64(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 
 hipError_t launch_rough_f32(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 grid, uint32_t payoff_items,
                             unsigned long long* error, hipStream_t s);
 hipError_t launch_rough_lsm_f64(const RoughLsmArgs& a, dim3 train_grid, dim3 value_grid, hipStream_t s,
-                               hipEvent_t paths_done, hipEvent_t between);
+                               hipEvent_t paths_done, hipEvent_t between, void* cash_copy);
 hipError_t launch_rough_lsm_f32(const RoughLsmArgs& a, dim3 train_grid, dim3 value_grid, hipStream_t s,
-                               hipEvent_t paths_done, hipEvent_t between);
+                               hipEvent_t paths_done, hipEvent_t between, void* cash_copy);
+hipError_t launch_rough_nn_value_f64(const RoughNnArgs& A, const RoughArgs& a, dim3 grid, hipStream_t s);
+hipError_t launch_rough_nn_value_f32(const RoughNnArgs& A, const RoughArgs& a, dim3 grid, hipStream_t s);
+hipError_t launch_rough_nn_date(const RoughNnArgs& A, uint32_t d, hipStream_t s);
+hipError_t launch_rough_nn_train(const RoughNnArgs& A, uint32_t d, uint32_t steps, uint32_t adam_t, hipStream_t s);
+hipError_t launch_rough_nn_final(const RoughNnArgs& A, hipStream_t s);
 hipError_t launch_lsm_dual_f64(const LsmDualArgs& da, uint64_t pairs, uint32_t outer_blocks, uint32_t basis,
                                hipStream_t s);
 hipError_t launch_lv_risk_f64(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s);

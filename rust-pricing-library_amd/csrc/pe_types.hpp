@@ -318,6 +318,42 @@ struct RoughLsmArgs {
     uint32_t pad;
 };
 
+// Network continuation for the Bermudan under lifted Heston: one 2-64-64-1
+// network per non-terminal date on (spot / K, variance), trained on the
+// FP64 training stores of RoughLsmArgs.
+struct RoughNnNorm {               // one date's row normalization
+    float mu_s, inv_s, mu_v, inv_v, mu_y, sd_y, inv_y;
+    uint32_t n;                    // in-the-money training rows
+};
+
+struct RoughNnArgs {
+    const uint32_t* ex_nodes;      // as RoughLsmArgs
+    const double* ex_discount;
+    const double* ex_scale;
+    const double* ex_times;        // year fraction of each exercise date
+    const double* store_s;         // FP64 training stores [date][path] (shared with the polynomial policy)
+    const double* store_v;
+    double* cash;                  // the network's own training cash flows x DF(stop) [path]
+    const float* init;             // initial parameters
+    float* weights;                // [non-terminal date][kMlpStride]
+    float* adam_m;                 // Adam moments [kMlpStride] (carried from date to date)
+    float* adam_v;
+    float4* rows;                  // (u_s, u_v, target, 1) per in-the-money training row
+    float* grad;                   // [gradient workgroup][parameters + 1]
+    double* stats;                 // [sweep workgroup][8]
+    uint32_t* offsets;             // [sweep workgroup]: first row of the workgroup's rows
+    RoughNnNorm* norms;            // [non-terminal date]
+    double* in_sample;
+    void* value_partials;          // BlockStat per valuation item (network policy)
+    unsigned long long* counts;    // valuation exercise counts [date] (network policy)
+    uint32_t* stops;               // optional: exercise-date index per valuation trajectory
+    unsigned long long* error;
+    uint64_t train_paths, value_chunk;
+    double strike, notional, itm_tol, kappa, theta;
+    float lr, beta1, beta2, eps;
+    uint32_t n_dates, call, n_wg, batch, epochs, grad_wgs, rows_per_wg, value_items;
+};
+
 // ------------------------------------------------------- early exercise --
 constexpr int kLsmThreads = 1024;     // induction workgroup: one per trade
 constexpr int kLsmMaxBasis = 4;       // spot feature, degree <= 3
@@ -439,7 +475,7 @@ struct MlpLsmArgs {
     float* weights;            // FP32 master parameters [trade][stride]
     float* adam_m;             // Adam moments [trade][stride]
     float* adam_v;
-    float4* rows;              // training rows (u1, u2, target, 0), one per training path of each trade
+    float4* rows;              // training rows (u1, u2, target, 1), one per training path of each trade
     LsmCuts* cuts;             // [trade][non-terminal date]
     unsigned int* error;       // bit 0: an exercise region needed more than kLsmMaxCuts cuts
     unsigned long long* rows_trained;   // [trade]: rows run through the network (epochs x rows, all dates)
