@@ -157,11 +157,15 @@ static void compile_plan(Plan& p, const pe_request& r, const pe_exec_policy* pol
     if (p.graph) {
         // the graph fixes the expiry (its last observation) and the
         // discounting date (its last payment)
-        if (r.model.kind == PE_MODEL_LOCAL_VOL) throw std::invalid_argument("graph products under local volatility are not restated");
         if (r.risk.delta || r.risk.vega || r.risk.gamma) throw std::invalid_argument("graph product Greeks are not restated");
         p.graph_payoff.compile(pr, [&](int32_t day) { return p.market.discount_curve.discount(static_cast<double>(day) / 365.0); });
         p.req.product.expiry_day = p.graph_payoff.expiry_day;
         p.req.product.payment_day = p.graph_payoff.last_payment;
+        // local-vol paths carry no discrete dividends here, so no observation
+        // has a pre-dividend spot (local_vol_valuation.rs:221-247 would fail
+        // the evaluation of such a node)
+        if (r.model.kind == PE_MODEL_LOCAL_VOL && !p.graph_payoff.pre_days.empty())
+            throw std::invalid_argument("pre-dividend graph observation under local volatility");
     }
     p.continuous = pr.kind == PE_PRODUCT_BARRIER && pr.continuous;
     if (p.continuous && r.model.kind == PE_MODEL_LOCAL_VOL)
@@ -729,6 +733,9 @@ static double lv_price(const Plan& p, const double* shocks) {
     lv_evolve(p, p.node_forwards, p.spot, shocks, tl_states);
     std::vector<double> post(p.obs_nodes.size());
     for (size_t i = 0; i < post.size(); ++i) post[i] = 0.0 * p.spot + 1.0 * tl_states[p.obs_nodes[i]];
+    // a graph product reads the same observations (payoff_outputs_from_local_vol_observations,
+    // local_vol_valuation.rs:221-247)
+    if (p.graph) return p.discount * p.graph_payoff.evaluate(post.data(), post.data(), p.obs_has_pre.data());
     p.tape.forward(post.data(), post.data(), tl_values);
     return p.discount * tl_values[p.tape.output];
 }

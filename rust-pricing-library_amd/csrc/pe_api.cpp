@@ -1287,6 +1287,8 @@ LvLaunch& prepare_lv(std::vector<HostPlan*>& plans, pe_backend backend) {
     if (plans.size() > static_cast<size_t>(kMaxProducts)) throw PlanError(PE_ERR_SIZE_LIMIT, "too many shared plans");
     for (HostPlan* q : plans) {
         if (!q->lv) throw PlanError(PE_ERR_INVALID_ARGUMENT, "shared evaluation needs local-vol plans");
+        if (q->graph && plans.size() > 1)
+            throw PlanError(PE_ERR_INVALID_ARGUMENT, "a graph product under local vol is evaluated on its own paths");
         if (q->risk && plans.size() > 1)
             throw PlanError(PE_ERR_INVALID_ARGUMENT, "shared evaluation needs price-only plans");
         if (q->nodes != p0.nodes || q->spot != p0.spot || q->node_forwards != p0.node_forwards ||
@@ -1308,7 +1310,8 @@ LvLaunch& prepare_lv(std::vector<HostPlan*>& plans, pe_backend backend) {
         for (size_t i = 0; i < q.obs_nodes.size(); ++i) {
             const uint32_t node = q.obs_nodes[i];
             node_weight[node * kMaxProducts + k] = q.obs_weight[i];
-            const bool needed = q.obs_weight[i] != 0.0 || static_cast<int32_t>(node) == q.product.terminal_obs;
+            // a graph reads every one of its observations, in order
+            const bool needed = q.graph || q.obs_weight[i] != 0.0 || static_cast<int32_t>(node) == q.product.terminal_obs;
             if (!needed) continue;
             if (node > 0) steps[node - 1].node_mask |= 1 << k;
             else a.node0_mask |= 1u << k;
@@ -1476,6 +1479,24 @@ LvLaunch& prepare_lv(std::vector<HostPlan*>& plans, pe_backend backend) {
         a.risk_lds_offset = static_cast<uint32_t>((L.lds + 15u) & ~static_cast<size_t>(15u));
         L.lds = a.risk_lds_offset + static_cast<size_t>(a.nt) * a.nx * real;
     }
+    if (p0.graph) {
+        // the graph's register program after the bridge stack: graph_regs
+        // values per lane
+        const GraphTape& gt = p0.graph_tape;
+        if (gt.segs.size() != p0.obs_nodes.size() + 2)
+            throw PlanError(PE_ERR_INVALID_ARGUMENT, "graph observations do not match the local-vol nodes");
+        L.graph = true;
+        L.d_graph_ops.upload(gt.ops.empty() ? std::vector<GraphOp>(1) : gt.ops);
+        L.d_graph_segs.upload(gt.segs);
+        L.d_graph_consts.upload(gt.consts.empty() ? std::vector<double>(1, 0.0) : gt.consts);
+        a.graph_ops = L.d_graph_ops.as<GraphOp>();
+        a.graph_segs = L.d_graph_segs.as<uint32_t>();
+        a.graph_consts = L.d_graph_consts.as<double>();
+        a.graph_output = gt.output;
+        a.graph_regs = std::max<uint32_t>(1, gt.regs);
+        a.graph_lds_offset = static_cast<uint32_t>((L.lds + 15u) & ~static_cast<size_t>(15u));
+        L.lds = a.graph_lds_offset + static_cast<size_t>(a.graph_regs) * kBlockThreads * real;
+    }
     if (L.lds > 160u * 1024u) throw PlanError(PE_ERR_SIZE_LIMIT, "local-vol grid and bridge stack exceed LDS");
     p0.lv_launch = std::move(launch);
     return *p0.lv_launch;
@@ -1491,6 +1512,10 @@ void evaluate_lv(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* o
             check(backend == PE_BACKEND_GPU_FP64 ? launch_lv_risk_f64(L.args, L.grid, L.lds, L.err.as<int>(), t_stream)
                                                  : launch_lv_risk_f32(L.args, L.grid, L.lds, L.err.as<int>(), t_stream),
                   "lv risk kernel launch");
+        else if (L.graph)
+            check(backend == PE_BACKEND_GPU_FP64 ? launch_lv_graph_f64(L.args, L.grid, L.lds, L.err.as<int>(), t_stream)
+                                                 : launch_lv_graph_f32(L.args, L.grid, L.lds, L.err.as<int>(), t_stream),
+                  "lv graph kernel launch");
         else
             check(backend == PE_BACKEND_GPU_FP64 ? launch_lv_f64(L.args, L.grid, L.lds, L.err.as<int>(), t_stream)
                                                  : launch_lv_f32(L.args, L.grid, L.lds, L.err.as<int>(), t_stream),

@@ -47,6 +47,10 @@ hipError_t launch_lsm_mlp_f64(const LsmArgs& train, uint32_t train_items, const 
     return launch_lsm_mlp<double>(train, train_items, m, judge, value, judge_value, value_items, risk, s, ev);
 }
 
+hipError_t launch_lv_graph_f64(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s) {
+    return launch_lv_graph<double>(a, grid, lds, err, s);
+}
+
 hipError_t launch_lv_f64(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s) {
     return launch_lv<double>(a, grid, lds, err, s);
 }

@@ -733,10 +733,12 @@ void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* p
     if (p.graph) {
         // the tape fixes the observation dates, the expiry (last observation)
         // and the discounting date (last payment)
-        if (r.model.kind == PE_MODEL_LOCAL_VOL) unsupported("graph products under local volatility");
         if (r.risk.delta || r.risk.vega || r.risk.gamma) unsupported("pathwise Greeks of graph products");
         compile_graph(pr, [&](int32_t day) { return std::exp(p.discount_curve.log_discount(static_cast<double>(day) / 365.0)); },
                       p.graph_tape);
+        // local-vol paths carry no discrete dividends: no pre-dividend spot
+        if (r.model.kind == PE_MODEL_LOCAL_VOL && !p.graph_tape.pre_days.empty())
+            unsupported("pre-dividend graph observation under local volatility");
         p.req.product.expiry_day = p.graph_tape.expiry_day;
         p.req.product.payment_day = p.graph_tape.last_payment;
     } else if (!(pr.strike > 0.0) || !(pr.notional > 0.0)) {

@@ -79,8 +79,10 @@ struct LvLaunch {
     LvArgs args{};
     dim3 grid;
     size_t lds = 0;
-    DeviceBuffer d_steps, d_node_weight, d_ops32, d_ops64, d_bins, d_bumps, partials, err;
+    DeviceBuffer d_steps, d_node_weight, d_ops32, d_ops64, d_bins, d_bumps, partials, err, d_graph_ops, d_graph_segs,
+        d_graph_consts;
     bool risk = false;               // one plan with Delta/Vega/Gamma: lv_risk_kernel
+    bool graph = false;              // one graph product: the graph flavour of lv_kernel
     void* host_partials = nullptr;   // pinned
     int* host_err = nullptr;         // pinned
     size_t partial_count = 0;
@@ -319,6 +321,8 @@ hipError_t launch_normal_mixed_error(uint64_t first, uint64_t count, uint32_t bl
                                      uint32_t* block_word, hipStream_t s);
 hipError_t launch_lv_f64(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s);
 hipError_t launch_lv_f32(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s);
+hipError_t launch_lv_graph_f64(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s);
+hipError_t launch_lv_graph_f32(const LvArgs& a, dim3 grid, size_t lds, int* err, hipStream_t s);
 hipError_t launch_dupire(const double* theta_t, const double* theta_v, const double* theta_d, uint32_t nk,
                          double slope, double rho, double eta, double gamma, const double* t, uint32_t nt,
                          const double* x, uint32_t nx, double floor, double cap, double* out, uint32_t* repairs,

@@ -529,6 +529,12 @@ struct LvArgs {
     uint32_t depth;
     uint32_t sobol_dims;
     uint32_t node0_mask;       // products with a fixing at the valuation node
+    // a graph product (one product per launch): its register program, one
+    // segment per observation, constants, output operand, registers in LDS
+    const GraphOp* graph_ops;
+    const uint32_t* graph_segs;
+    const double* graph_consts;
+    uint32_t graph_output, graph_regs, graph_lds_offset, graph_pad;
     // risk kernel only (one plan): bumped runtimes and the node-tangent table
     const LvBumpStep* bump_steps;
     double spot_dn, spot_up, spot_bump;

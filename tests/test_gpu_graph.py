@@ -1,7 +1,8 @@
-"""GPU parity of graph products (the BS batch kernel's register tape)
-against the oracle's lazy graph evaluation: FP64 device mode within 1e-12
-of the oracle (and of the replay fixtures' prices for the graph forms of
-their products), the mixed mode within 0.1 standard errors; the device's
+"""GPU parity of graph products (the register tape in the BS batch kernel
+and in the graph flavour of the local-vol kernel) against the oracle's lazy
+graph evaluation: FP64 device mode within 1e-12 of the oracle (and of the
+replay fixtures' prices for the graph forms of their products, Black-Scholes
+and local vol), the mixed mode within 0.1 standard errors; the device's
 poison bits raise exactly where the reference's lazy evaluation fails."""
 import copy
 
@@ -110,3 +111,50 @@ def test_pre_dividend_spot_on_an_ex_date():
     call = g.push("maximum", (g.push("subtract", (term, g.lit(100.0))), g.lit(0.0)))
     out = g.push("add", (call, drop))
     _check(G.graph_request(doc, g, out, 240))
+
+
+# ---- graph products on local-vol paths (the graph flavour of lv_kernel) ----
+
+@pytest.mark.parametrize("name", ["pseudo_mc_price_only", "rqmc_price_only"])
+def test_local_vol_fixture_products_as_graphs(name):
+    """The local-vol replay fixtures' European as a graph: the FP64 device
+    mode reproduces the oracle and the fixture's recorded price."""
+    from test_oracle_graph import fixture_graph
+
+    case, doc = G.replay_request(name, "local_volatility")
+    g, out, pay = fixture_graph(doc)
+    dev = _check(G.graph_request(doc, g, out, pay), block=case["reduction_block_size"])
+    assert _close(dev.estimates[0].value, case["value"]["value"])
+
+
+def test_local_vol_path_dependent_graphs():
+    """Monthly Asian, lookback and barrier graphs on local-vol paths: FP64
+    device = oracle within 1e-12, mixed within 0.1 SE; the graph Asian equals
+    the fixed Asian on the same device paths in FP64 mode."""
+    _, doc = G.replay_request("pseudo_mc_price_only", "local_volatility")
+    days = [30 * (i + 1) for i in range(12)] + [365]
+    w = [1.0 / 13] * 13
+    for g, o in (G.asian(days, w, 100.0), G.lookback(days, 95.0, call=False),
+                 G.barrier_out(days[-1], days, 100.0, 85.0, down=True)):
+        _check(G.graph_request(doc, g, o, days[-1]))
+    g, o = G.asian(days, w, 100.0)
+    graph = engine.PricingPlan.compile(G.graph_request(doc, g, o, days[-1])).evaluate("fp64")
+    plain = request.Request.from_v3(copy.deepcopy(doc))
+    plain.c.risk.delta = plain.c.risk.vega = plain.c.risk.gamma = 0
+    plain.set_product(abi.PE_PRODUCT_ASIAN, abi.PE_CALL, days[-1], 100.0, dates=days, weights=w)
+    fixed = engine.PricingPlan.compile(plain).evaluate("fp64")
+    assert _close(graph.estimates[0].value, fixed.estimates[0].value)
+
+
+def test_local_vol_graph_non_finite_value_fails():
+    """A graph whose evaluated node is non-finite (log of a negative
+    difference) fails the unit under local vol in both modes."""
+    _, doc = G.replay_request("pseudo_mc_price_only", "local_volatility")
+    g = G.G()
+    s = g.spot(365)
+    bad = g.push("log", (g.push("subtract", (s, g.lit(1e9))),))
+    plan = engine.PricingPlan.compile(G.graph_request(doc, g, bad, 365))
+    for mode in ("fp64", "mixed"):
+        with pytest.raises(engine.EngineError) as err:
+            plan.evaluate(mode)
+        assert err.value.status == abi.PE_ERR_NONFINITE_STATE

@@ -247,3 +247,53 @@ def test_coupon_and_capped_call_document_matches_quadrature():
     exact = 5.0 * coupon * _discount(doc, p1) + spread * _discount(doc, p2)
     est = res.estimates[0]
     assert abs(est.value - exact) < 4.0 * est.standard_error, (est.value, exact, est.standard_error)
+
+
+# ---- graph products on local-vol paths (local_vol_valuation.rs:221-247) ----
+
+LV_FIXTURES = ["pseudo_mc_price_only", "rqmc_price_only"]
+
+
+@pytest.mark.parametrize("name", LV_FIXTURES)
+def test_graph_european_under_local_vol_reproduces_the_replay_fixture(name):
+    case, doc = G.replay_request(name, "local_volatility")
+    g, out, pay = fixture_graph(doc)
+    res = oracle_evaluate(G.graph_request(doc, g, out, pay), workers=4, block=case["reduction_block_size"])
+    exp = case["value"]
+    assert abs(res.estimates[0].value - exp["value"]) <= 1e-12 * abs(exp["value"])
+    assert abs(res.estimates[0].standard_error - exp["standard_error"]) <= 1e-10 * exp["standard_error"]
+
+
+def test_graph_path_dependence_under_local_vol_equals_the_fixed_products():
+    """Monthly Asian, lookback and barrier graphs ending at the local-vol
+    grid's expiry (day 365) price exactly as the fixed products on the same
+    local-vol paths."""
+    _, doc = G.replay_request("pseudo_mc_price_only", "local_volatility")
+    days = [30 * (i + 1) for i in range(12)] + [365]
+    w = [1.0 / 13] * 13
+    g, o = G.asian(days, w, 100.0)
+    _fixed_vs_graph(doc, lambda r: r.set_product(abi.PE_PRODUCT_ASIAN, abi.PE_CALL, days[-1], 100.0, dates=days,
+                                                 weights=w), g, o, days[-1])
+    g, o = G.lookback(days, 95.0, call=False)
+    _fixed_vs_graph(doc, lambda r: r.set_product(abi.PE_PRODUCT_LOOKBACK, abi.PE_PUT, days[-1], 95.0, dates=days),
+                    g, o, days[-1])
+    g, o = G.barrier_out(days[-1], days, 100.0, 85.0, down=True)
+    _fixed_vs_graph(doc, lambda r: r.set_product(abi.PE_PRODUCT_BARRIER, abi.PE_CALL, days[-1], 100.0, barrier=85.0,
+                                                 direction_down=1, dates=days), g, o, days[-1])
+
+
+def test_pre_dividend_spot_under_local_vol_is_refused():
+    """Local-vol paths carry no discrete dividends here: a graph reading a
+    pre-dividend spot is refused at compile time (oracle and engine)."""
+    from pricing_amd import engine
+    _, doc = G.replay_request("pseudo_mc_price_only", "local_volatility")
+    g = G.G()
+    s = g.pre_spot(365)
+    k = g.lit(100.0)
+    out = g.push("maximum", (g.push("subtract", (s, k)), g.lit(0.0)))
+    req = G.graph_request(doc, g, out, 365)
+    with pytest.raises(RuntimeError):
+        oracle_evaluate(req, workers=1)
+    with pytest.raises(engine.EngineError) as err:
+        engine.PricingPlan.compile(req)
+    assert err.value.status == abi.PE_ERR_UNSUPPORTED
