@@ -157,7 +157,8 @@ static void compile_plan(Plan& p, const pe_request& r, const pe_exec_policy* pol
     if (p.graph) {
         // the graph fixes the expiry (its last observation) and the
         // discounting date (its last payment)
-        if (r.risk.delta || r.risk.vega || r.risk.gamma) throw std::invalid_argument("graph product Greeks are not restated");
+        if (r.model.kind == PE_MODEL_LOCAL_VOL && (r.risk.delta || r.risk.vega || r.risk.gamma))
+            throw std::invalid_argument("graph product Greeks under local volatility are not restated");
         p.graph_payoff.compile(pr, [&](int32_t day) { return p.market.discount_curve.discount(static_cast<double>(day) / 365.0); });
         p.req.product.expiry_day = p.graph_payoff.expiry_day;
         p.req.product.payment_day = p.graph_payoff.last_payment;
@@ -606,7 +607,7 @@ static void continuous_price_values(const Plan& p, const double* z, double* v) {
 // pathwise_aad (black_scholes.rs:476-563), generic observation loop.
 static Aad aad_generic(const Plan& p, const double* z, double spot, double vol) {
     if (p.continuous) return aad_continuous(p, z, spot, vol);
-    if (p.single_terminal) return aad_single_terminal(p, z[0], spot, vol);
+    if (p.single_terminal && !p.graph) return aad_single_terminal(p, z[0], spot, vol);
     bs_observations(p, z, spot, vol, tl_obs);
     const size_t n = tl_obs.size();
     std::vector<double> post(n), pre(n), post_adj(n, 0.0), pre_adj(n, 0.0);
@@ -614,8 +615,16 @@ static Aad aad_generic(const Plan& p, const double* z, double spot, double vol) 
         post[i] = tl_obs[i].post;
         pre[i] = tl_obs[i].pre;
     }
-    p.tape.forward(post.data(), pre.data(), tl_values);
-    p.tape.reverse(tl_values, post_adj.data(), pre_adj.data());
+    double value;
+    if (p.graph) {
+        // the graph's lazy evaluation and reverse sweep
+        value = p.graph_payoff.evaluate_adjoints(post.data(), pre.data(), p.obs_has_pre.data(), post_adj.data(),
+                                                 pre_adj.data());
+    } else {
+        p.tape.forward(post.data(), pre.data(), tl_values);
+        p.tape.reverse(tl_values, post_adj.data(), pre_adj.data());
+        value = tl_values[p.tape.output];
+    }
     double delta = 0.0, vega = 0.0;
     for (size_t i = 0; i < n; ++i) {
         if (post_adj[i] == 0.0) continue;
@@ -629,7 +638,7 @@ static Aad aad_generic(const Plan& p, const double* z, double spot, double vol) 
         delta += pre_adj[i] * c.spot_scale * tl_obs[i].f / spot;
         vega += pre_adj[i] * c.b * tl_obs[i].f * (-vol * p.obs_times[i] + tl_obs[i].w);
     }
-    return {p.discount * tl_values[p.tape.output], p.discount * delta, p.discount * vega};
+    return {p.discount * value, p.discount * delta, p.discount * vega};
 }
 
 // discounted_payoff_from_normals (black_scholes.rs:41-56)

@@ -254,8 +254,105 @@ struct GraphPayoff {
     // Lazy evaluation of the output on one path: post[i] / pre[i] are the
     // spots at observation i; has_pre[i] marks a pre-dividend spot.
     double evaluate(const double* post, const double* pre, const char* has_pre) const {
-        std::vector<double> v(nodes.size(), 0.0);
-        std::vector<char> done(nodes.size(), 0);
+        std::vector<double> v;
+        std::vector<char> done;
+        return run(post, pre, has_pre, v, done);
+    }
+
+    // evaluate_single_with_observation_adjoints (tape.rs:484-600): the lazy
+    // evaluation, then the reverse sweep over the evaluated nodes in reverse
+    // order with reverse_opcode's pullbacks (tape.rs:602-780); post_adj[i] /
+    // pre_adj[i] (zeroed by the caller) receive d output / d spot at
+    // observation i.  A non-finite adjoint is an error (add_adjoint).
+    double evaluate_adjoints(const double* post, const double* pre, const char* has_pre, double* post_adj,
+                             double* pre_adj) const {
+        std::vector<double> v;
+        std::vector<char> done;
+        const double value = run(post, pre, has_pre, v, done);
+        std::vector<double> adj(nodes.size(), 0.0);
+        adj[output] = 1.0;
+        auto add = [&](int slot, double c) {
+            const double u = adj[slot] + c;
+            if (!std::isfinite(u)) throw std::runtime_error("non-finite graph runtime adjoint");
+            adj[slot] = u;
+        };
+        for (int id = static_cast<int>(nodes.size()) - 1; id >= 0; --id) {
+            if (!done[id]) continue;
+            const GNode& g = nodes[id];
+            const double o = adj[id];
+            if (g.constant) continue;   // a literal or folded value has no inputs
+            const double x = g.arity > 0 ? v[g.in[0]] : 0.0, y = g.arity > 1 ? v[g.in[1]] : 0.0;
+            const Smooth sm{g.value};
+            switch (g.op) {
+                case PE_GRAPH_TERMINAL_SPOT: post_adj[g.obs] += o; break;
+                case PE_GRAPH_PRE_DIVIDEND_SPOT: pre_adj[g.obs] += o; break;
+                case PE_GRAPH_SELECT: add(x != 0.0 ? g.in[1] : g.in[2], o); break;
+                case PE_GRAPH_ADD: add(g.in[0], o); add(g.in[1], o); break;
+                case PE_GRAPH_SUBTRACT: add(g.in[0], o); add(g.in[1], -o); break;
+                case PE_GRAPH_MULTIPLY: add(g.in[0], o * y); add(g.in[1], o * x); break;
+                case PE_GRAPH_DIVIDE: add(g.in[0], o / y); add(g.in[1], denominator_adjoint(o, x, y)); break;
+                case PE_GRAPH_MINIMUM: add(x <= y ? g.in[0] : g.in[1], o); break;
+                case PE_GRAPH_MAXIMUM: add(x >= y ? g.in[0] : g.in[1], o); break;
+                case PE_GRAPH_NEGATE: add(g.in[0], -o); break;
+                case PE_GRAPH_LOG: add(g.in[0], o / x); break;
+                case PE_GRAPH_EXP: add(g.in[0], o * v[id]); break;
+                case PE_GRAPH_BARRIER_HIT_STATE: add(g.in[0], o * (1.0 - y)); add(g.in[1], o * (1.0 - x)); break;
+                case PE_GRAPH_SMOOTH_MAXIMUM: {
+                    double val, lf, rf;
+                    sm.maximum(x, y, &val, &lf, &rf);
+                    add(g.in[0], o * lf);
+                    add(g.in[1], o * rf);
+                    break;
+                }
+                case PE_GRAPH_SMOOTH_MINIMUM: {
+                    // SmoothingPolicy::minimum (product/smoothing.rs:123-145)
+                    const double d = x - y;
+                    double lf, rf;
+                    if (d >= sm.h) { lf = 0.0; rf = 1.0; }
+                    else if (d <= -sm.h) { lf = 1.0; rf = 0.0; }
+                    else {
+                        double pv, pf, comp;
+                        sm.positive_part(d, &pv, &pf, &comp);
+                        lf = comp;
+                        rf = pf;
+                    }
+                    add(g.in[0], o * lf);
+                    add(g.in[1], o * rf);
+                    break;
+                }
+                case PE_GRAPH_SMOOTH_INDICATOR: {
+                    double val, first;
+                    sm.indicator(x, &val, &first);
+                    add(g.in[0], o * first);
+                    break;
+                }
+                default: break;   // Compare, Indicator, Literal: no derivative
+            }
+        }
+        return value;
+    }
+
+    // division_denominator_adjoint (tape.rs:783-805): -seed n / d^2 without
+    // losing a finite value when the product or the square leaves the normal
+    // range (fractions in [1, 2) and one final scaling).
+    static double denominator_adjoint(double seed, double n, double d) {
+        const double product = -seed * n, square = d * d;
+        if (std::isnormal(square) && std::isnormal(product)) return product / square;
+        if (seed == 0.0 || n == 0.0) return product;
+        auto parts = [](double x, int* e) {
+            const double m = std::frexp(x, e);   // m in [0.5, 1)
+            *e -= 1;
+            return 2.0 * m;                      // in [1, 2)
+        };
+        int es, en, ed;
+        const double fs = parts(seed, &es), fn = parts(n, &en), fd = parts(d, &ed);
+        return std::ldexp(-fs * fn / (fd * fd), es + en - 2 * ed);
+    }
+
+    double run(const double* post, const double* pre, const char* has_pre, std::vector<double>& v,
+               std::vector<char>& done) const {
+        v.assign(nodes.size(), 0.0);
+        done.assign(nodes.size(), 0);
         struct Frame { int id, stage; };
         std::vector<Frame> st{{output, 0}};
         while (!st.empty()) {

@@ -107,9 +107,11 @@ def replay_request(name, source=""):
 
 
 def graph_request(doc, g, output, payment_day, risk=False):
-    """`doc`'s market, model and engine with the graph as the product."""
+    """`doc`'s market, model and engine with the graph as the product; with
+    `risk`, also `doc`'s Greek requests."""
     plain = copy.deepcopy(doc)
-    plain["risk"] = {"delta": False, "vega": False}
+    if not risk:
+        plain["risk"] = {"delta": False, "vega": False}
     req = request.Request.from_v3(plain)
     req.set_graph_product(g.nodes, [(output, payment_day)], [U])
     return req
