@@ -234,8 +234,16 @@ float timed(F&& launch) {
 // longest-first; each has a fixed partial slot so the per-trade merge below
 // is in unit order whatever the schedule.
 constexpr uint64_t kItemUnits = 4096;
-// groups: bit 0 risk, bit 1 continuous barrier; group 4: graph products
-constexpr int kBsGroups = 5;
+// groups: bit 0 risk, bit 1 continuous barrier; group 4: graph products,
+// group 5: graph products with Greeks
+constexpr int kBsGroups = 6;
+
+int bs_group_of(const HostPlan& p) {
+    if (p.graph) return p.risk ? 5 : 4;
+    return (p.risk ? 1 : 0) | (p.continuous ? 2 : 0);
+}
+bool bs_group_risk(int g) { return g == 5 || (g < 4 && (g & 1) != 0); }
+bool bs_group_cont(int g) { return g < 4 && (g & 2) != 0; }
 
 }  // namespace
 
@@ -265,7 +273,7 @@ struct BsLaunch {
     size_t partial_count = 0;
     // persistent grid size per group, chosen by the tuner on the first
     // evaluation; the tuned launches are then captured in one HIP graph
-    uint32_t grid[kBsGroups] = {0, 0, 0, 0, 0};
+    uint32_t grid[kBsGroups] = {};
     bool tuned = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -294,11 +302,11 @@ BsLaunch& prepare_bs(std::vector<HostPlan*>& plans, pe_backend backend) {
     std::vector<uint32_t> graph_segs;
     std::vector<double> graph_consts;
     for (int g = 0; g < kBsGroups; ++g) {
-        L->groups[g].risk = g < 4 && (g & 1) != 0;
-        L->groups[g].cont = g < 4 && (g & 2) != 0;
+        L->groups[g].risk = bs_group_risk(g);
+        L->groups[g].cont = bs_group_cont(g);
         L->groups[g].components = L->groups[g].cont ? PE_COMPONENTS : (L->groups[g].risk ? 10u : 1u);
     }
-    uint32_t group_partials[kBsGroups] = {0, 0, 0, 0, 0};
+    uint32_t group_partials[kBsGroups] = {};
     for (size_t ti = 0; ti < plans.size(); ++ti) {
         HostPlan& p = *plans[ti];
         if (p.lv) throw PlanError(PE_ERR_INVALID_ARGUMENT, "local-vol plans are evaluated by pe_plans_evaluate_shared");
@@ -322,10 +330,11 @@ BsLaunch& prepare_bs(std::vector<HostPlan*>& plans, pe_backend backend) {
             graph_segs.insert(graph_segs.end(), gt.segs.begin(), gt.segs.end());
             graph_consts.insert(graph_consts.end(), gt.consts.begin(), gt.consts.end());
             if (gt.segs.size() != p.bs_obs.size() + 2) throw PlanError(PE_ERR_INVALID_ARGUMENT, "graph schedule");
-            L->groups[4].graph_regs = std::max<uint32_t>(L->groups[4].graph_regs, gt.regs);
+            uint32_t& regs = L->groups[bs_group_of(p)].graph_regs;
+            regs = std::max<uint32_t>(regs, gt.regs);
         }
         L->trades.push_back(t);
-        const int g = p.graph ? 4 : (p.risk ? 1 : 0) | (p.continuous ? 2 : 0);
+        const int g = bs_group_of(p);
         BsGroup& grp = L->groups[g];
         BsPlanSlot slot;
         slot.group = g;
@@ -393,10 +402,11 @@ BsBatchArgs bs_args(BsLaunch& L, int g, uint32_t grid) {
     a.n_items = static_cast<uint32_t>(grp.items.size());
     a.components = grp.components;
     a.grid = grid;
+    a.graph_regs = grp.graph_regs;
     return a;
 }
 
-uint32_t enqueue_bs_batch(BsLaunch& L, pe_backend backend, const uint32_t (&grid)[kBsGroups], uint32_t mask = 31u) {
+uint32_t enqueue_bs_batch(BsLaunch& L, pe_backend backend, const uint32_t (&grid)[kBsGroups], uint32_t mask = 63u) {
     uint32_t launches = 0;
     for (int g = 0; g < kBsGroups; ++g) {
         BsGroup& grp = L.groups[g];
@@ -433,7 +443,7 @@ void tune_bs_batch(BsLaunch& L, pe_backend backend) {
         for (uint32_t per_cu : {0u, 4u, 8u, 16u}) {
             const uint32_t candidate = per_cu * static_cast<uint32_t>(cus);
             if (per_cu && candidate >= n) continue;
-            uint32_t grid[kBsGroups] = {0, 0, 0, 0, 0};
+            uint32_t grid[kBsGroups] = {};
             grid[g] = candidate;
             const float ms = timed([&] { enqueue_bs_batch(L, backend, grid, 1u << g); });
             if (per_cu == 0) best = ms;
@@ -520,8 +530,8 @@ void evaluate_bs_noalloc(HostPlan& p, pe_backend backend, pe_result* out) {
     const pe_engine& e = p.req.engine;
     const bool rqmc = e.kind == PE_ENGINE_RQMC;
     const uint32_t scrambles = rqmc ? e.scrambles : 1u;
-    const int g = p.graph ? 4 : (p.risk ? 1 : 0) | (p.continuous ? 2 : 0);
-    const bool risk = g < 4 && (g & 1) != 0, cont = g < 4 && (g & 2) != 0;
+    const int g = bs_group_of(p);
+    const bool risk = bs_group_risk(g), cont = bs_group_cont(g);
     const uint32_t comps = cont ? PE_COMPONENTS : (risk ? 10u : 1u);
     const uint64_t chunks64 = (e.units + kItemUnits - 1) / kItemUnits;
     const uint64_t n_items64 = chunks64 * scrambles;
@@ -592,6 +602,7 @@ void evaluate_bs_noalloc(HostPlan& p, pe_backend backend, pe_result* out) {
     a.components = comps;
     a.grid = 0;
     const uint32_t graph_regs = p.graph ? gt.regs : 0u;
+    a.graph_regs = graph_regs;
     check(backend == PE_BACKEND_GPU_FP64 ? launch_bs_batch_f64(a, risk, cont, graph_regs, t_stream)
                                          : launch_bs_batch_f32(a, risk, cont, graph_regs, t_stream),
           "bs kernel launch");

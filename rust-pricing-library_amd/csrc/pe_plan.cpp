@@ -733,9 +733,14 @@ void compile_host_plan(HostPlan& p, const pe_request& r, const pe_exec_policy* p
     if (p.graph) {
         // the tape fixes the observation dates, the expiry (last observation)
         // and the discounting date (last payment)
-        if (r.risk.delta || r.risk.vega || r.risk.gamma) unsupported("pathwise Greeks of graph products");
+        const bool greeks = r.risk.delta || r.risk.vega || r.risk.gamma;
+        if (greeks && r.model.kind == PE_MODEL_LOCAL_VOL) unsupported("pathwise Greeks of graph products under local volatility");
         compile_graph(pr, [&](int32_t day) { return std::exp(p.discount_curve.log_discount(static_cast<double>(day) / 365.0)); },
                       p.graph_tape);
+        // Greeks keep each register's two tangents beside its value in LDS
+        if (greeks && p.graph_tape.regs > kGraphMaxRiskRegs)
+            unsupported("pathwise Greeks of a graph product needing more than " + std::to_string(kGraphMaxRiskRegs) +
+                        " registers");
         // local-vol paths carry no discrete dividends: no pre-dividend spot
         if (r.model.kind == PE_MODEL_LOCAL_VOL && !p.graph_tape.pre_days.empty())
             unsupported("pre-dividend graph observation under local volatility");

@@ -22,6 +22,8 @@ enum ProductKind : int32_t { kEuropean = 0, kDigital = 1, kBarrier = 2, kAsian =
 // on), values in kGraphMaxRegs registers per lane, constants in a table.
 // An operand below kGraphConst is a register, otherwise a constant index.
 constexpr uint32_t kGraphMaxRegs = 32;
+// with pathwise Greeks each register also carries two tangents in LDS
+constexpr uint32_t kGraphMaxRiskRegs = 20;
 constexpr uint16_t kGraphConst = 0x8000u;
 // operands read straight from the current observation (a spot node whose
 // every consumer runs in its own observation's segment is not materialised)
@@ -217,7 +219,7 @@ struct BsBatchArgs {
     uint32_t n_items;
     uint32_t components;       // statistics per item (1, 10 or 18)
     uint32_t grid;             // persistent workgroups (0: one per item)
-    uint32_t pad;
+    uint32_t graph_regs;       // graph groups: LDS registers per lane
 };
 
 // ------------------------------------------------------- lifted Heston --

@@ -158,3 +158,91 @@ def test_local_vol_graph_non_finite_value_fails():
         with pytest.raises(engine.EngineError) as err:
             plan.evaluate(mode)
         assert err.value.status == abi.PE_ERR_NONFINITE_STATE
+
+
+# ---- pathwise Greeks of graph products (forward tangents in LDS) ----
+
+def _check_risk(req, block=0, mixed=True):
+    """All ten risk components (price, pathwise Delta / Vega, Gamma from the
+    bumped Deltas, the bump validations) of the FP64 device mode against the
+    oracle's reverse sweep, 1e-12 of each component's conditioning (a
+    validation component divides price differences by the bump); the mixed
+    mode's price, Delta and Vega within 0.1 standard errors."""
+    from test_gpu_parity import compare
+
+    ref = oracle_evaluate(req, workers=4, block=block)
+    plan = engine.PricingPlan.compile(req, request.exec_policy(1, block) if block else None)
+    dev = plan.evaluate("fp64")
+    for c in range(10):
+        print("graph risk", abi.COMPONENT_NAMES[c], ref.estimates[c].value, dev.estimates[c].value, flush=True)
+    compare(dev, ref, list(range(10)), req=req)
+    if mixed:
+        mx = plan.evaluate("mixed")
+        for c in (0, 1, 2):
+            se = ref.estimates[c].standard_error
+            assert abs(mx.estimates[c].value - ref.estimates[c].value) <= 0.1 * se + 1e-12, abi.COMPONENT_NAMES[c]
+    return dev
+
+
+@pytest.mark.parametrize("name", ["pseudo_mc_full_risk", "rqmc_full_risk"])
+def test_fixture_greeks_as_graphs(name):
+    from test_oracle_graph import fixture_graph
+
+    case, doc = G.replay_request(name)
+    g, out, pay = fixture_graph(doc)
+    dev = _check_risk(G.graph_request(doc, g, out, pay, risk=True), block=case["reduction_block_size"], mixed=False)
+    for key, index in (("delta", 1), ("vega", 2), ("gamma", 3)):
+        assert _close(dev.estimates[index].value, case["risks"][key]["value"]), key
+
+
+@pytest.mark.parametrize("engine_kind", ["pseudo", "rqmc"])
+def test_path_dependent_graph_greeks(engine_kind):
+    _, doc = G.replay_request("pseudo_mc_full_risk" if engine_kind == "pseudo" else "rqmc_full_risk")
+    days = [30 * (i + 1) for i in range(12)]
+    for g, o in (G.asian(days, [1.0 / 12] * 12, 100.0), G.lookback(days, 95.0, call=False),
+                 G.barrier_out(days[-1], days, 100.0, 85.0, down=True)):
+        _check_risk(G.graph_request(doc, g, o, days[-1], risk=True))
+
+
+def test_smooth_graph_greeks():
+    """divide, log, exp, negate, the smooth operations and a select."""
+    from test_oracle_graph import smooth_graph
+
+    _, doc = G.replay_request("pseudo_mc_full_risk")
+    g, out, pay = smooth_graph()
+    _check_risk(G.graph_request(doc, g, out, pay, risk=True))
+
+
+def test_pre_dividend_spot_greeks():
+    """Tangents of a pre-dividend spot through its own affine coordinate."""
+    _, doc = G.replay_request("pseudo_mc_full_risk")
+    doc = copy.deepcopy(doc)
+    doc["market"]["discrete_dividends"] = [{"ex_time": 120 / 365, "quote": {"type": "fixed_cash", "amount": 2.0}}]
+    g = G.G()
+    pre = g.pre_spot(120)
+    post = g.spot(120)
+    term = g.spot(240)
+    ratio = g.push("divide", (post, pre))
+    call = g.push("maximum", (g.push("subtract", (term, g.lit(100.0))), g.lit(0.0)))
+    out = g.push("add", (g.push("multiply", (call, ratio)), pre))
+    _check_risk(G.graph_request(doc, g, out, 240, risk=True))
+
+
+def test_graph_greeks_in_a_batch_equal_single_plans():
+    """A batch with graph products priced and with Greeks beside a fixed
+    product: each result bit-identical to its plan evaluated alone."""
+    from test_oracle_graph import smooth_graph
+
+    _, doc = G.replay_request("pseudo_mc_full_risk")
+    days = [30 * (i + 1) for i in range(6)]
+    reqs = [G.graph_request(doc, *G.asian(days, [1.0 / 6] * 6, 100.0), days[-1], risk=True),
+            G.graph_request(doc, *G.asian(days, [1.0 / 6] * 6, 100.0), days[-1]),
+            G.graph_request(doc, *smooth_graph(), risk=True),
+            request.Request.from_v3(copy.deepcopy(doc))]
+    plans = [engine.PricingPlan.compile(r) for r in reqs]
+    batch = engine.evaluate_batch(plans, "fp64")
+    for p, b in zip(plans, batch):
+        solo = p.evaluate("fp64")
+        for c in range(10):
+            assert b.estimates[c].value == solo.estimates[c].value
+            assert b.estimates[c].standard_error == solo.estimates[c].standard_error

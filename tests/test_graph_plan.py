@@ -63,8 +63,23 @@ def test_live_value_limit(doc):
     assert _compile_status(G.graph_request(doc, g, acc, 400)) == abi.PE_OK
 
 
-def test_graph_greeks_are_unsupported(doc):
+def test_graph_greeks_compile_within_the_tangent_register_limit(doc):
     g, o = G.european(180, 100.0)
-    req = G.graph_request(doc, g, o, 180)
+    assert _compile_status(G.graph_request(doc, g, o, 180, risk=True)) == abi.PE_OK
+    # 24 observations live until one final sum: within the 32 registers of a
+    # price, beyond the 20 that carry two tangents each for Greeks
+    g = G.G()
+    spots = [g.spot(10 * (i + 1)) for i in range(24)]
+    acc = spots[-1]
+    for s in reversed(spots[:-1]):
+        acc = g.push("add", (s, acc))
+    assert _compile_status(G.graph_request(doc, g, acc, 240)) == abi.PE_OK
+    assert _compile_status(G.graph_request(doc, g, acc, 240, risk=True)) == abi.PE_ERR_UNSUPPORTED
+
+
+def test_graph_greeks_under_local_vol_are_unsupported():
+    _, doc = G.replay_request("pseudo_mc_price_only", "local_volatility")
+    g, o = G.european(365, 100.0)
+    req = G.graph_request(doc, g, o, 365)
     req.c.risk.delta = 1
     assert _compile_status(req) == abi.PE_ERR_UNSUPPORTED
