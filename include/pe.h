@@ -119,10 +119,13 @@ typedef enum pe_product_kind {
     PE_PRODUCT_AMERICAN = 5,    /* AmericanVanillaSpec: exercise dates in dates[], the last at expiry */
     PE_PRODUCT_GRAPH = 6        /* GraphProduct (crates/pricing/src/product/graph_product.rs:40-191):
                                    SSA payoff graph + cash flows in graph_nodes[] / graph_cashflows[];
-                                   Black-Scholes / Black-76 models, price only (Greeks and local
-                                   volatility: PE_ERR_UNSUPPORTED); at most 32 values live at once
-                                   after scheduling by observation (PE_ERR_SIZE_LIMIT); a live
-                                   non-finite value fails the valuation (PE_ERR_NONFINITE_STATE) */
+                                   Black-Scholes / Black-76 models with pathwise Delta / Vega /
+                                   Gamma (at most 20 live values with Greeks, else
+                                   PE_ERR_UNSUPPORTED); local volatility price only (Greeks:
+                                   PE_ERR_UNSUPPORTED); at most 32 values live at once after
+                                   scheduling by observation (PE_ERR_SIZE_LIMIT); a live
+                                   non-finite value or Greek fails the valuation
+                                   (PE_ERR_NONFINITE_STATE) */
 } pe_product_kind;
 
 /* SourceOpcode (crates/pricing/src/product/graph.rs:89-215); the JSON names of
