@@ -1395,32 +1395,49 @@ LvLaunch& prepare_lv(std::vector<HostPlan*>& plans, pe_backend backend) {
             default: h.mode = kModeTerminal;
         }
     }
-    // Uniform-bin cell table for the mixed kernel: the smallest power-of-two
-    // bin count (up to 4096) with at most one interior node per bin.
+    // Uniform-bin cell table for the mixed kernel.  The kernel maps the
+    // clamped FP32 coordinate xx to bin b = trunc((xx - x0) * inv_bin_w) in
+    // FP32; the same map is evaluated here on the FP32 nodes.  It is monotone
+    // in xx, so bin_cell[b] = the number of interior nodes whose image lies
+    // below b is a lower bound of xx's cell, and at most the next node (whose
+    // image may equal b) remains to be compared.  The bin count is the
+    // smallest power of two (up to 4096) at which no two interior nodes share
+    // an image; the table covers every image up to that of the last node.
     std::vector<uint16_t> bin_cell;
     {
         const std::vector<double>& xs = p0.lv_x;
-        const double x0 = xs.front(), xl = xs.back();
+        const size_t nx = xs.size();
+        std::vector<float> xf(nx);
+        for (size_t i = 0; i < nx; ++i) xf[i] = static_cast<float>(xs[i]);
+        const float x0f = xf.front(), xlf = xf.back();
         for (uint32_t m = 64; m <= 4096 && bin_cell.empty(); m *= 2) {
-            const double w = (xl - x0) / m;
-            bool ok = true;
-            std::vector<uint16_t> t(m);
-            for (uint32_t b = 0; b < m && ok; ++b) {
-                const double lo = x0 + w * b, hi = x0 + w * (b + 1);
-                uint32_t inside = 0;
-                for (size_t i = 0; i < xs.size(); ++i) inside += (xs[i] > lo && xs[i] < hi) ? 1u : 0u;
-                ok = inside <= 1;
-                size_t c = 0;
-                while (c + 1 < xs.size() - 1 && xs[c + 1] <= lo) ++c;
-                t[b] = static_cast<uint16_t>(c);
+            const float inv = static_cast<float>(m / (xs.back() - xs.front()));
+            auto image = [&](float x) -> int32_t {
+                const volatile float d = x - x0f;   // the kernel's two FP32 roundings, in order
+                const volatile float q = d * inv;
+                return static_cast<int32_t>(q);
+            };
+            const int32_t last = image(xlf);
+            bool ok = last >= 0 && last < 65536;
+            std::vector<int32_t> img;
+            for (size_t j = 1; ok && j + 1 < nx; ++j) {
+                img.push_back(image(xf[j]));
+                ok = img.size() < 2 || img[img.size() - 1] > img[img.size() - 2];
             }
-            if (ok) bin_cell.swap(t);
+            if (!ok) continue;
+            std::vector<uint16_t> t(static_cast<size_t>(last) + 1);
+            size_t below = 0;   // interior nodes with an image < b
+            for (int32_t b = 0; b <= last; ++b) {
+                while (below < img.size() && img[below] < b) ++below;
+                t[static_cast<size_t>(b)] = static_cast<uint16_t>(below);
+            }
+            bin_cell.swap(t);
+            a.inv_bin_w32 = inv;
         }
         if (bin_cell.empty()) throw PlanError(PE_ERR_SIZE_LIMIT, "log-moneyness nodes too uneven for the cell table");
         a.bins = static_cast<uint32_t>(bin_cell.size());
-        a.inv_bin_w32 = static_cast<float>(a.bins / (xl - x0));
-        a.x0_32 = static_cast<float>(x0);
-        a.xl_32 = static_cast<float>(xl);
+        a.x0_32 = x0f;
+        a.xl_32 = xlf;
     }
     DeviceBuffer& d_steps = L.d_steps;
     DeviceBuffer& d_node_weight = L.d_node_weight;
@@ -1468,8 +1485,13 @@ LvLaunch& prepare_lv(std::vector<HostPlan*>& plans, pe_backend backend) {
     // LDS: grid, x nodes and inverse widths (Real), the per-wave Sobol words
     // (RQMC), the bin table (mixed), then the bridge stack (one slot per lane
     // per tree level).
+    // (mixed: the cells' 16-byte records and the grid by column, columns of
+    // nt | 1 words)
     const size_t real = backend == PE_BACKEND_GPU_FP64 ? sizeof(double) : sizeof(float);
-    const size_t head = real * (a.nt * a.nx + 2u * a.nx) + (a.rqmc ? 4u * sizeof(uint32_t) * a.sobol_dims : 0u);
+    const size_t grid_bytes = backend == PE_BACKEND_GPU_FP64
+                                  ? real * (a.nt * a.nx + 2u * a.nx)
+                                  : 16u * a.nx + sizeof(float) * static_cast<size_t>(a.nx) * (a.nt | 1u);
+    const size_t head = grid_bytes + (a.rqmc ? 4u * sizeof(uint32_t) * a.sobol_dims : 0u);
     const size_t bin_bytes = backend == PE_BACKEND_GPU_FP64 ? 0u : ((a.bins * sizeof(uint16_t) + 15u) & ~static_cast<size_t>(15u));
     L.lds = ((head + 15u) & ~static_cast<size_t>(15u)) + bin_bytes +
             static_cast<size_t>(std::max<uint32_t>(a.depth, 1u)) * kBlockThreads * real;
