@@ -156,7 +156,7 @@ void ensure_init() {
 }
 
 // Converts block partials to per-component statistics and fills the result.
-void assemble(const HostPlan& p, const std::vector<BlockStat>& partials, uint32_t blocks_x, uint32_t scrambles,
+void assemble(const HostPlan& p, const BlockStat* partials, uint32_t blocks_x, uint32_t scrambles,
               uint32_t comps, uint32_t comp_offset, uint32_t result_comps, pe_result* out) {
     const pe_engine& e = p.req.engine;
     const bool rqmc = e.kind == PE_ENGINE_RQMC;
@@ -199,6 +199,11 @@ void assemble(const HostPlan& p, const std::vector<BlockStat>& partials, uint32_
     out->diagnostics = p.diag;
 }
 
+void assemble(const HostPlan& p, const std::vector<BlockStat>& partials, uint32_t blocks_x, uint32_t scrambles,
+              uint32_t comps, uint32_t comp_offset, uint32_t result_comps, pe_result* out) {
+    assemble(p, partials.data(), blocks_x, scrambles, comps, comp_offset, result_comps, out);
+}
+
 dim3 grid_for(const pe_engine& e) {
     const uint64_t blocks = (e.units + kBlockThreads - 1) / kBlockThreads;
     const uint32_t bx = static_cast<uint32_t>(std::min<uint64_t>(blocks, 1u << 16));
@@ -209,6 +214,24 @@ void ensure_sobol(HostPlan& p) {
     const pe_engine& e = p.req.engine;
     if (e.kind != PE_ENGINE_RQMC || p.d_sobol.ptr) return;
     p.d_sobol.upload(build_sobol_table(e.seed, e.scrambles, p.sobol_dims));
+}
+
+// The two timing events of this thread's calls, created once.
+struct CallEvents {
+    hipEvent_t start = nullptr, stop = nullptr;
+    CallEvents() {
+        check(hipEventCreate(&start), "hipEventCreate");
+        check(hipEventCreate(&stop), "hipEventCreate");
+    }
+    ~CallEvents() {
+        if (start) (void)hipEventDestroy(start);
+        if (stop) (void)hipEventDestroy(stop);
+    }
+};
+
+CallEvents& call_events() {
+    thread_local CallEvents ev;
+    return ev;
 }
 
 template <typename F>
@@ -1396,8 +1419,9 @@ LvLaunch& prepare_lv(std::vector<HostPlan*>& plans, pe_backend backend) {
         }
     }
     // Uniform-bin cell table for the mixed kernel.  The kernel maps the
-    // clamped FP32 coordinate xx to bin b = trunc((xx - x0) * inv_bin_w) in
-    // FP32; the same map is evaluated here on the FP32 nodes.  It is monotone
+    // clamped FP32 coordinate xx to bin b = trunc(fma(xx, inv_bin_w,
+    // -x0 inv_bin_w)) in FP32; the same map is evaluated here on the FP32
+    // nodes.  It is monotone
     // in xx, so bin_cell[b] = the number of interior nodes whose image lies
     // below b is a lower bound of xx's cell, and at most the next node (whose
     // image may equal b) remains to be compared.  The bin count is the
@@ -1412,9 +1436,9 @@ LvLaunch& prepare_lv(std::vector<HostPlan*>& plans, pe_backend backend) {
         const float x0f = xf.front(), xlf = xf.back();
         for (uint32_t m = 64; m <= 4096 && bin_cell.empty(); m *= 2) {
             const float inv = static_cast<float>(m / (xs.back() - xs.front()));
+            const float off = -(x0f * inv);
             auto image = [&](float x) -> int32_t {
-                const volatile float d = x - x0f;   // the kernel's two FP32 roundings, in order
-                const volatile float q = d * inv;
+                const volatile float q = std::fmaf(x, inv, off);   // the kernel's single rounding
                 return static_cast<int32_t>(q);
             };
             const int32_t last = image(xlf);
@@ -1433,6 +1457,7 @@ LvLaunch& prepare_lv(std::vector<HostPlan*>& plans, pe_backend backend) {
             }
             bin_cell.swap(t);
             a.inv_bin_w32 = inv;
+            a.bin_off32 = off;
         }
         if (bin_cell.empty()) throw PlanError(PE_ERR_SIZE_LIMIT, "log-moneyness nodes too uneven for the cell table");
         a.bins = static_cast<uint32_t>(bin_cell.size());
@@ -1540,7 +1565,11 @@ void evaluate_lv(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* o
     const int no_error = 0x7fffffff;
     *L.host_err = no_error;
     check(hipMemcpyAsync(L.err.ptr, L.host_err, sizeof(int), hipMemcpyHostToDevice, t_stream), "error word init");
-    const float ms = timed([&] {
+    // launch, the copies back and one synchronisation; the kernel's time comes
+    // from the events around the launch alone
+    CallEvents& ev = call_events();
+    check(hipEventRecord(ev.start, t_stream), "hipEventRecord");
+    {
         if (L.risk)
             check(backend == PE_BACKEND_GPU_FP64 ? launch_lv_risk_f64(L.args, L.grid, L.lds, L.err.as<int>(), t_stream)
                                                  : launch_lv_risk_f32(L.args, L.grid, L.lds, L.err.as<int>(), t_stream),
@@ -1553,19 +1582,21 @@ void evaluate_lv(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* o
             check(backend == PE_BACKEND_GPU_FP64 ? launch_lv_f64(L.args, L.grid, L.lds, L.err.as<int>(), t_stream)
                                                  : launch_lv_f32(L.args, L.grid, L.lds, L.err.as<int>(), t_stream),
                   "lv kernel launch");
-    });
+    }
+    check(hipEventRecord(ev.stop, t_stream), "hipEventRecord");
     check(hipMemcpyAsync(L.host_partials, L.partials.ptr, L.partial_count * sizeof(BlockStat), hipMemcpyDeviceToHost,
                          t_stream), "partials copy");
     check(hipMemcpyAsync(L.host_err, L.err.ptr, sizeof(int), hipMemcpyDeviceToHost, t_stream), "error word copy");
     check(hipStreamSynchronize(t_stream), "stream synchronize");
+    float ms = 0.0f;
+    check(hipEventElapsedTime(&ms, ev.start, ev.stop), "hipEventElapsedTime");
     const int failing = *L.host_err;
     if (failing != no_error) {
         for (size_t k = 0; k < plans.size(); ++k) outs[k].failing_unit = static_cast<uint64_t>(failing);
         throw PlanError(PE_ERR_NONFINITE_STATE,
                         "local-vol Log-Euler produced a non-finite state at sampling unit " + std::to_string(failing));
     }
-    const BlockStat* host = static_cast<const BlockStat*>(L.host_partials);
-    const std::vector<BlockStat> partials(host, host + L.partial_count);
+    const BlockStat* partials = static_cast<const BlockStat*>(L.host_partials);
     for (size_t k = 0; k < plans.size(); ++k) {
         std::memset(&outs[k], 0, sizeof(pe_result));
         if (L.risk) {

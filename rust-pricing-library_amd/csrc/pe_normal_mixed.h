@@ -20,11 +20,12 @@
 
 namespace pe {
 
-// p - 1/2 and s = min(p, 1 - p) of the word's midpoint probability, exact in
-// integer arithmetic before one FP32 rounding (no word maps to 0, 1 or the
-// wrong side of 1/2).
+// 2^32 (p - 1/2) and s = min(p, 1 - p) of the word's midpoint probability,
+// exact in integer arithmetic before one FP32 rounding (no word maps to 0, 1
+// or the wrong side of 1/2).  The centred value is left scaled by 2^32: the
+// power of two is folded into nm_central's constant, which rounds the same.
 PE_NM_HD float nm_centered(uint32_t word) {
-    return (static_cast<float>(static_cast<int32_t>(word ^ 0x80000000u)) + 0.5f) * 2.3283064365386963e-10f;
+    return static_cast<float>(static_cast<int32_t>(word ^ 0x80000000u)) + 0.5f;
 }
 
 PE_NM_HD float nm_tail_probability(uint32_t word) {
@@ -32,7 +33,7 @@ PE_NM_HD float nm_tail_probability(uint32_t word) {
     return (static_cast<float>(low) + 0.5f) * 2.3283064365386963e-10f;
 }
 
-// w < 5: z = 2 sqrt(2) (p - 1/2) p(w - 2.5)
+// w < 5: z = 2 sqrt(2) (p - 1/2) p(w - 2.5), with centered = 2^32 (p - 1/2)
 PE_NM_HD float nm_central(float centered, float w) {
     const float v = w - 2.5f;
     float p = -9.315122007e-09f;
@@ -45,7 +46,7 @@ PE_NM_HD float nm_central(float centered, float w) {
     p = p * v - 4.177368246e-03f;
     p = p * v + 2.466402650e-01f;
     p = p * v + 1.501409411e+00f;
-    return (2.8284271247461903f * centered) * p;
+    return ((2.8284271247461903f * 2.3283064365386963e-10f) * centered) * p;
 }
 
 // w >= 5: z = sqrt(2) (1 - 2 s) p(sqrt(w) - 3), signed by the word's half

@@ -504,11 +504,13 @@ struct LvArgs {
     uint32_t n_ops;
     uint32_t pad_ops;
     ProductHot hot[kMaxProducts];
-    // Uniform bins over [x0, xl]: bin_cell[b] is the grid cell of the bin's
-    // left edge; every bin holds at most one interior node (mixed kernel).
+    // Uniform bins over [x0, xl] (mixed kernel): the FP32 coordinate xx maps
+    // to bin trunc(fma(xx, inv_bin_w32, bin_off32)); bin_cell[b] counts the
+    // interior nodes whose own bin is below b, and no two interior nodes
+    // share a bin.
     const uint16_t* bin_cell;
     uint32_t bins;
-    float inv_bin_w32, x0_32, xl_32;
+    float inv_bin_w32, bin_off32, x0_32, xl_32;
     const double* grid;        // [nt][nx] local variance (FP64)
     const float* grid32;       // FP32 copy for the mixed kernel
     const double* x_nodes;
