@@ -462,6 +462,22 @@ void compile_bs_trade(HostPlan& p, double monitoring_end) {
         }
     }
     t.n_variants = nv;
+    // per-observation coordinates at the spot-bumped variants, with the
+    // kernel's at_spot arithmetic (cash held fixed: dividend.rs:163-173)
+    for (BsObs& o : p.bs_obs) {
+        for (int k = 0; k < 2; ++k) {
+            const double spot = nv >= 3 ? t.variants[k + 1].spot : p.spot;
+            const double original = p.spot;
+            if (original == spot) {
+                o.va[k] = o.a; o.vb[k] = o.b; o.vpa[k] = o.pa; o.vpb[k] = o.pb;
+            } else {
+                o.va[k] = o.a * original / spot;
+                o.vb[k] = o.ss - (o.ss - o.b) * original / spot;
+                o.vpa[k] = o.pa * original / spot;
+                o.vpb[k] = o.pss - (o.pss - o.pb) * original / spot;
+            }
+        }
+    }
 }
 
 // ------------------------------------------------------------ time grid --

@@ -83,6 +83,10 @@ struct BsObs {
     double time, sqrt_step, forward, weight;
     double a, b, ss;          // post-dividend coordinate (A, B, spot scale)
     double pa, pb, pss;       // pre-dividend coordinate
+    // the coordinates refitted at the spot-bumped risk variants 1 and 2
+    // (AffineDividendCoordinate::at_spot; the other variants use the base
+    // spot): post (va, vb) and pre (vpa, vpb), spot scales unchanged
+    double va[2], vb[2], vpa[2], vpb[2];
     uint32_t flags;
     uint32_t pad;
 };
