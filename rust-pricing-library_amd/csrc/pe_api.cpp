@@ -259,14 +259,26 @@ float timed(F&& launch) {
 constexpr uint64_t kItemUnits = 4096;
 // groups: bit 0 risk, bit 1 continuous barrier; group 4: graph products,
 // group 5: graph products with Greeks
-constexpr int kBsGroups = 6;
+// Launch groups: fixed products with discrete monitoring by product family
+// (European, Digital, Barrier, Asian, Lookback) and price / risk, so each
+// launch runs one family's payoff code; continuous barriers; graph products.
+constexpr int kBsFamilies = 5;
+constexpr int kBsContGroup = 2 * kBsFamilies;
+constexpr int kBsGraphGroup = kBsContGroup + 2;
+constexpr int kBsGroups = kBsGraphGroup + 2;
 
 int bs_group_of(const HostPlan& p) {
-    if (p.graph) return p.risk ? 5 : 4;
-    return (p.risk ? 1 : 0) | (p.continuous ? 2 : 0);
+    const int risk = p.risk ? 1 : 0;
+    if (p.graph) return kBsGraphGroup + risk;
+    if (p.continuous) return kBsContGroup + risk;
+    const int kind = p.product.kind;
+    if (kind < 0 || kind >= kBsFamilies) throw PlanError(PE_ERR_INVALID_ARGUMENT, "product family of a batch trade");
+    return 2 * kind + risk;
 }
-bool bs_group_risk(int g) { return g == 5 || (g < 4 && (g & 1) != 0); }
-bool bs_group_cont(int g) { return g < 4 && (g & 2) != 0; }
+bool bs_group_risk(int g) { return (g & 1) != 0; }
+bool bs_group_cont(int g) { return g == kBsContGroup || g == kBsContGroup + 1; }
+// the product family compiled into the group's kernel (-1: any)
+int bs_group_kind(int g) { return g < kBsContGroup ? g / 2 : -1; }
 
 }  // namespace
 
@@ -291,7 +303,7 @@ struct BsLaunch {
     std::vector<BsTrade> trades;
     std::vector<BsPlanSlot> slots;
     BsGroup groups[kBsGroups];
-    DeviceBuffer d_trades, d_obs, d_sobol, partials, err, d_graph_ops, d_graph_segs, d_graph_consts;
+    DeviceBuffer d_trades, d_obs, d_obs32, d_sobol, partials, err, d_graph_ops, d_graph_segs, d_graph_consts;
     std::vector<BlockStat> host_partials;
     size_t partial_count = 0;
     // persistent grid size per group, chosen by the tuner on the first
@@ -320,6 +332,7 @@ BsLaunch& prepare_bs(std::vector<HostPlan*>& plans, pe_backend backend) {
     L->key = key;
     L->backend = static_cast<int>(backend);
     std::vector<BsObs> obs;
+    std::vector<BsObs32> obs32;
     std::vector<uint32_t> sobol;
     std::vector<GraphOp> graph_ops;
     std::vector<uint32_t> graph_segs;
@@ -336,6 +349,7 @@ BsLaunch& prepare_bs(std::vector<HostPlan*>& plans, pe_backend backend) {
         BsTrade t = p.trade;
         t.obs_offset = static_cast<uint32_t>(obs.size());
         obs.insert(obs.end(), p.bs_obs.begin(), p.bs_obs.end());
+        obs32.insert(obs32.end(), p.bs_obs32.begin(), p.bs_obs32.end());
         const pe_engine& e = p.req.engine;
         const uint32_t scrambles = e.kind == PE_ENGINE_RQMC ? e.scrambles : 1u;
         if (e.kind == PE_ENGINE_RQMC) {
@@ -395,6 +409,7 @@ BsLaunch& prepare_bs(std::vector<HostPlan*>& plans, pe_backend backend) {
     L->partial_count = base;
     L->d_trades.upload(L->trades);
     L->d_obs.upload(obs);
+    L->d_obs32.upload(obs32);
     if (!sobol.empty()) L->d_sobol.upload(sobol);
     if (!graph_ops.empty()) {
         L->d_graph_ops.upload(graph_ops);
@@ -415,6 +430,7 @@ BsBatchArgs bs_args(BsLaunch& L, int g, uint32_t grid) {
     BsBatchArgs a{};
     a.trades = L.d_trades.as<BsTrade>();
     a.obs = L.d_obs.as<BsObs>();
+    a.obs32 = L.d_obs32.as<BsObs32>();
     a.sobol = L.d_sobol.as<uint32_t>();
     a.items = grp.d_items.as<BsItem>();
     a.partials = static_cast<BlockStat*>(L.partials.ptr) + grp.partial_base;
@@ -429,14 +445,17 @@ BsBatchArgs bs_args(BsLaunch& L, int g, uint32_t grid) {
     return a;
 }
 
-uint32_t enqueue_bs_batch(BsLaunch& L, pe_backend backend, const uint32_t (&grid)[kBsGroups], uint32_t mask = 63u) {
+uint32_t enqueue_bs_batch(BsLaunch& L, pe_backend backend, const uint32_t (&grid)[kBsGroups],
+                          uint32_t mask = ~0u) {
     uint32_t launches = 0;
     for (int g = 0; g < kBsGroups; ++g) {
         BsGroup& grp = L.groups[g];
         if (grp.items.empty() || !((mask >> g) & 1u)) continue;
         const BsBatchArgs a = bs_args(L, g, grid[g]);
-        check(backend == PE_BACKEND_GPU_FP64 ? launch_bs_batch_f64(a, grp.risk, grp.cont, grp.graph_regs, t_stream)
-                                             : launch_bs_batch_f32(a, grp.risk, grp.cont, grp.graph_regs, t_stream),
+        const int kind = bs_group_kind(g);
+        check(backend == PE_BACKEND_GPU_FP64
+                  ? launch_bs_batch_f64(a, grp.risk, grp.cont, kind, grp.graph_regs, t_stream)
+                  : launch_bs_batch_f32(a, grp.risk, grp.cont, kind, grp.graph_regs, t_stream),
               "bs kernel launch");
         ++launches;
     }
@@ -492,8 +511,10 @@ void evaluate_bs_batch(std::vector<HostPlan*>& plans, pe_backend backend, pe_res
             if (L.groups[g].items.empty()) continue;
             const BsBatchArgs a = bs_args(L, g, L.grid[g]);
             const BsGroup& grp = L.groups[g];
-            check(backend == PE_BACKEND_GPU_FP64 ? add_bs_batch_node_f64(L.graph, a, grp.risk, grp.cont, grp.graph_regs)
-                                                 : add_bs_batch_node_f32(L.graph, a, grp.risk, grp.cont, grp.graph_regs),
+            const int kind = bs_group_kind(g);
+            check(backend == PE_BACKEND_GPU_FP64
+                      ? add_bs_batch_node_f64(L.graph, a, grp.risk, grp.cont, kind, grp.graph_regs)
+                      : add_bs_batch_node_f32(L.graph, a, grp.risk, grp.cont, kind, grp.graph_regs),
                   "graph kernel node");
         }
         check(hipGraphInstantiate(&L.graph_exec, L.graph, nullptr, nullptr, 0), "graph instantiate");
@@ -565,6 +586,7 @@ void evaluate_bs_noalloc(HostPlan& p, pe_backend backend, pe_result* out) {
     const size_t o_err = off; off = align(off + sizeof(unsigned long long));
     const size_t o_trade = off; off = align(off + sizeof(BsTrade));
     const size_t o_obs = off; off = align(off + p.bs_obs.size() * sizeof(BsObs));
+    const size_t o_obs32 = off; off = align(off + p.bs_obs32.size() * sizeof(BsObs32));
     const size_t o_sobol = off; off = align(off + p.host_sobol.size() * sizeof(uint32_t));
     const GraphTape& gt = p.graph_tape;
     const size_t o_gops = off; off = align(off + (p.graph ? gt.ops.size() * sizeof(GraphOp) : 0));
@@ -604,6 +626,7 @@ void evaluate_bs_noalloc(HostPlan& p, pe_backend backend, pe_result* out) {
     put(o_err, &none, sizeof(none));
     put(o_trade, &t, sizeof(t));
     put(o_obs, p.bs_obs.data(), p.bs_obs.size() * sizeof(BsObs));
+    put(o_obs32, p.bs_obs32.data(), p.bs_obs32.size() * sizeof(BsObs32));
     put(o_sobol, p.host_sobol.data(), p.host_sobol.size() * sizeof(uint32_t));
     if (p.graph) {
         put(o_gops, gt.ops.data(), gt.ops.size() * sizeof(GraphOp));
@@ -614,6 +637,7 @@ void evaluate_bs_noalloc(HostPlan& p, pe_backend backend, pe_result* out) {
     BsBatchArgs a{};
     a.trades = reinterpret_cast<const BsTrade*>(base + o_trade);
     a.obs = reinterpret_cast<const BsObs*>(base + o_obs);
+    a.obs32 = reinterpret_cast<const BsObs32*>(base + o_obs32);
     a.sobol = reinterpret_cast<const uint32_t*>(base + o_sobol);
     a.items = reinterpret_cast<const BsItem*>(base + o_items);
     a.partials = base + o_partials;
@@ -626,8 +650,9 @@ void evaluate_bs_noalloc(HostPlan& p, pe_backend backend, pe_result* out) {
     a.grid = 0;
     const uint32_t graph_regs = p.graph ? gt.regs : 0u;
     a.graph_regs = graph_regs;
-    check(backend == PE_BACKEND_GPU_FP64 ? launch_bs_batch_f64(a, risk, cont, graph_regs, t_stream)
-                                         : launch_bs_batch_f32(a, risk, cont, graph_regs, t_stream),
+    const int kind = bs_group_kind(g);
+    check(backend == PE_BACKEND_GPU_FP64 ? launch_bs_batch_f64(a, risk, cont, kind, graph_regs, t_stream)
+                                         : launch_bs_batch_f32(a, risk, cont, kind, graph_regs, t_stream),
           "bs kernel launch");
     std::vector<BlockStat> partials(static_cast<size_t>(n_items) * comps);
     unsigned long long err = none;

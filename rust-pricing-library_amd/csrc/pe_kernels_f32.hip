@@ -58,12 +58,14 @@ hipError_t launch_normal_mixed_error(uint64_t first, uint64_t count, uint32_t bl
     return hipGetLastError();
 }
 
-hipError_t add_bs_batch_node_f32(hipGraph_t graph, const BsBatchArgs& a, bool risk, bool cont, uint32_t graph_regs) {
-    return add_bs_batch_node<float>(graph, a, risk, cont, graph_regs);
+hipError_t add_bs_batch_node_f32(hipGraph_t graph, const BsBatchArgs& a, bool risk, bool cont, int kind,
+                                 uint32_t graph_regs) {
+    return add_bs_batch_node<float>(graph, a, risk, cont, kind, graph_regs);
 }
 
-hipError_t launch_bs_batch_f32(const BsBatchArgs& a, bool risk, bool cont, uint32_t graph_regs, hipStream_t s) {
-    return launch_bs_batch<float>(a, risk, cont, graph_regs, s);
+hipError_t launch_bs_batch_f32(const BsBatchArgs& a, bool risk, bool cont, int kind, uint32_t graph_regs,
+                               hipStream_t s) {
+    return launch_bs_batch<float>(a, risk, cont, kind, graph_regs, s);
 }
 
 hipError_t launch_rough_lsm_f32(const RoughLsmArgs& a, dim3 train_grid, dim3 value_grid, hipStream_t s,

@@ -14,12 +14,14 @@
 
 namespace pe {
 
-hipError_t add_bs_batch_node_f64(hipGraph_t graph, const BsBatchArgs& a, bool risk, bool cont, uint32_t graph_regs) {
-    return add_bs_batch_node<double>(graph, a, risk, cont, graph_regs);
+hipError_t add_bs_batch_node_f64(hipGraph_t graph, const BsBatchArgs& a, bool risk, bool cont, int kind,
+                                 uint32_t graph_regs) {
+    return add_bs_batch_node<double>(graph, a, risk, cont, kind, graph_regs);
 }
 
-hipError_t launch_bs_batch_f64(const BsBatchArgs& a, bool risk, bool cont, uint32_t graph_regs, hipStream_t s) {
-    return launch_bs_batch<double>(a, risk, cont, graph_regs, s);
+hipError_t launch_bs_batch_f64(const BsBatchArgs& a, bool risk, bool cont, int kind, uint32_t graph_regs,
+                               hipStream_t s) {
+    return launch_bs_batch<double>(a, risk, cont, kind, graph_regs, s);
 }
 
 hipError_t launch_rough_lsm_f64(const RoughLsmArgs& a, dim3 train_grid, dim3 value_grid, hipStream_t s,

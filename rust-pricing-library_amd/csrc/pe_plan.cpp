@@ -448,7 +448,10 @@ void compile_bs_trade(HostPlan& p, double monitoring_end) {
     t.risk = p.risk ? 1u : 0u;
     t.graph_output = p.graph_tape.output;
     t.graph_regs = p.graph_tape.regs;
-    auto variant = [&](int k, double spot, double vol) { t.variants[k] = Variant{spot, vol, spot / p.spot}; };
+    auto variant = [&](int k, double spot, double vol) {
+        t.variants[k] = Variant{spot, vol, spot / p.spot, static_cast<float>(spot), static_cast<float>(vol),
+                                static_cast<float>(vol * 1.4426950408889634), static_cast<float>(1.0 / spot)};
+    };
     variant(0, p.spot, p.vol);
     uint32_t nv = 1;
     if (p.risk) {
@@ -476,6 +479,59 @@ void compile_bs_trade(HostPlan& p, double monitoring_end) {
                 o.vpa[k] = o.pa * original / spot;
                 o.vpb[k] = o.pss - (o.pss - o.pb) * original / spot;
             }
+        }
+    }
+    // FP32 constants of the mixed kernel, from the FP64 values above
+    const double log2e = 1.4426950408889634;
+    const double spots[3] = {p.spot, nv >= 3 ? t.variants[1].spot : p.spot, nv >= 3 ? t.variants[2].spot : p.spot};
+    const double vols[3] = {p.vol, nv >= 5 ? t.variants[3].vol : p.vol, nv >= 5 ? t.variants[4].vol : p.vol};
+    const double barrier = p.product.barrier;
+    auto f32 = [](double x) { return static_cast<float>(x); };
+    p.bs_obs32.assign(p.bs_obs.size(), BsObs32{});
+    double prev_bridge = 0.0;   // time of the previous bridge observation (valuation: 0)
+    for (size_t i = 0; i < p.bs_obs.size(); ++i) {
+        const BsObs& o = p.bs_obs[i];
+        BsObs32& q = p.bs_obs32[i];
+        q.flags = o.flags;
+        q.time = f32(o.time);
+        q.sqrt_step = f32(o.sqrt_step);
+        q.weight = f32(o.weight);
+        q.sqrt_time = f32(std::sqrt(o.time));
+        q.ss = f32(o.ss);
+        q.pss = f32(o.pss);
+        for (int si = 0; si < 3; ++si) {
+            const double spot = spots[si];
+            const double A = si ? o.va[si - 1] : o.a, B = si ? o.vb[si - 1] : o.b;
+            const double PA = si ? o.vpa[si - 1] : o.pa, PB = si ? o.vpb[si - 1] : o.pb;
+            q.fwd[si] = f32(o.forward * (spot / p.spot));
+            q.cash[si] = f32(A * spot);
+            q.b[si] = f32(B);
+            q.pcash[si] = f32(PA * spot);
+            q.pb[si] = f32(PB);
+            q.ssos[si] = f32(o.ss / spot);
+            q.pssos[si] = f32(o.pss / spot);
+            q.a[si] = f32(A);
+            q.pa[si] = f32(PA);
+            const double bar = B != 0.0 ? (barrier - A * spot) / B : 0.0;
+            const double pbar = PB != 0.0 ? (barrier - PA * spot) / PB : 0.0;
+            q.bar[si] = f32(bar);
+            q.pbar[si] = f32(pbar);
+            q.itsb[si] = B * bar != 0.0 ? f32(1.0 / (B * bar)) : 0.0f;
+            q.pitsb[si] = PB * pbar != 0.0 ? f32(1.0 / (PB * pbar)) : 0.0f;
+        }
+        const bool bridge = (o.flags & kObsBridge) != 0u;
+        const double dt = bridge ? o.time - prev_bridge : 0.0;
+        if (bridge) prev_bridge = o.time;
+        for (int vi = 0; vi < 3; ++vi) {
+            const double vol = vols[vi];
+            q.mh2[vi] = f32(-0.5 * (vol * vol * o.time) * log2e);
+            q.nvt[vi] = f32(-vol * o.time);
+            q.sd2[vi] = f32(vol * std::sqrt(o.time) * log2e);
+            const double var = vol * vol;
+            const double iv = (0.5 * var + 0.5 * var) * dt;
+            q.iv[vi] = f32(iv);
+            q.liv[vi] = iv > 0.0 ? f32(std::log(iv)) : 0.0f;
+            q.iiv[vi] = iv > 0.0 ? f32(1.0 / iv) : 0.0f;
         }
     }
 }

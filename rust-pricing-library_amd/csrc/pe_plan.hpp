@@ -222,6 +222,7 @@ struct HostPlan {
     // BS/Black-76 trade descriptor and observation table for the batch kernel
     BsTrade trade{};
     std::vector<BsObs> bs_obs;
+    std::vector<BsObs32> bs_obs32;        // FP32 constants of the mixed batch kernel
 
     // early exercise (American product): exercise dates and the LSM runtime
     bool american = false;
@@ -280,11 +281,15 @@ hipError_t scratch_setup(int device);   // with `device` current; once per devic
 void* scratch_pointer(int device);
 
 // Launchers (pe_kernels_f64.hip / pe_kernels_f32.hip).
-hipError_t launch_bs_batch_f64(const BsBatchArgs& a, bool risk, bool cont, uint32_t graph_regs, hipStream_t s);
-hipError_t launch_bs_batch_f32(const BsBatchArgs& a, bool risk, bool cont, uint32_t graph_regs, hipStream_t s);
+hipError_t launch_bs_batch_f64(const BsBatchArgs& a, bool risk, bool cont, int kind, uint32_t graph_regs,
+                               hipStream_t s);
+hipError_t launch_bs_batch_f32(const BsBatchArgs& a, bool risk, bool cont, int kind, uint32_t graph_regs,
+                               hipStream_t s);
 // The same launch as a kernel node of `graph` (no dependencies).
-hipError_t add_bs_batch_node_f64(hipGraph_t graph, const BsBatchArgs& a, bool risk, bool cont, uint32_t graph_regs);
-hipError_t add_bs_batch_node_f32(hipGraph_t graph, const BsBatchArgs& a, bool risk, bool cont, uint32_t graph_regs);
+hipError_t add_bs_batch_node_f64(hipGraph_t graph, const BsBatchArgs& a, bool risk, bool cont, int kind,
+                                 uint32_t graph_regs);
+hipError_t add_bs_batch_node_f32(hipGraph_t graph, const BsBatchArgs& a, bool risk, bool cont, int kind,
+                                 uint32_t graph_regs);
 hipError_t launch_rough_f64(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 grid, uint32_t payoff_items,
                             unsigned long long* error, hipStream_t s);
 hipError_t launch_rough_f32(const RoughArgs& a, const RoughPayoffArgs& pa, dim3 grid, uint32_t payoff_items,

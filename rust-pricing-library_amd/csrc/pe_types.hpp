@@ -97,6 +97,35 @@ struct Variant {
     double spot;
     double vol;
     double spot_scale;   // spot / S0
+    float f_spot, f_vol;  // FP32 copies for the mixed kernels
+    float f_vol2;         // vol log2(e): the exponent's Brownian term in base 2
+    float f_inv_spot;     // 1 / spot
+};
+
+// FP32 per-observation constants of the mixed batch kernel, computed in FP64
+// and rounded once.  Spot variants si: 0 base, 1 S - h, 2 S + h (the risk
+// variants 1 and 2); vol variants vi: 0 base, 1 sigma - hv, 2 sigma + hv
+// (the risk variants 3 and 4); every other risk variant uses index 0.
+struct BsObs32 {
+    float time, sqrt_step, weight, sqrt_time;
+    float fwd[3];            // forward x spot scale
+    float cash[3], b[3];     // post-dividend S = cash + B F: cash = A spot
+    float pcash[3], pb[3];   // pre-dividend
+    float ssos[3], pssos[3]; // spot scale / spot (d S / d S0 per unit F), post and pre
+    float ss, pss;
+    float a[3], pa[3];       // A at the spot variants (continuous barrier states)
+    float bar[3], pbar[3];   // (barrier - A spot) / B, post and pre (continuous barrier)
+    float mh2[3];            // -sigma^2 t / 2 log2(e)
+    float nvt[3];            // -sigma t
+    float sd2[3];            // sigma sqrt(t) log2(e) (single terminal step)
+    // continuous barrier, per vol variant: the variance of the bridge interval
+    // ending here (sigma^2 times the time since the previous bridge
+    // observation), its log and reciprocal; per spot variant: 1 / (B x the
+    // transformed barrier), post and pre
+    float iv[3], liv[3], iiv[3];
+    float itsb[3], pitsb[3];
+    uint32_t flags;          // BsObsFlags
+    float pad;
 };
 
 // Streaming bridge program: PUSH computes the value at a split node from the
@@ -213,6 +242,7 @@ struct BsItem {
 struct BsBatchArgs {
     const BsTrade* trades;
     const BsObs* obs;
+    const BsObs32* obs32;      // the same observations' FP32 constants (mixed kernels)
     const uint32_t* sobol;
     const BsItem* items;
     void* partials;            // BlockStat [partial][component]
