@@ -166,13 +166,21 @@ void assemble(const HostPlan& p, const BlockStat* partials, uint32_t blocks_x, u
         for (uint32_t c = 0; c < result_comps; ++c)
             for (uint32_t b = 0; b < blocks_x; ++b) stats[c].merge(partials[static_cast<size_t>(b) * comps + comp_offset + c]);
     } else {
+        // only each scramble's compensated total is used: the blocks' sums
+        // are added in the same order as Stat::merge adds them, without its
+        // centred moments
+        std::vector<double> means(scrambles);
         for (uint32_t c = 0; c < result_comps; ++c) {
-            std::vector<double> means;
             for (uint32_t s = 0; s < scrambles; ++s) {
                 Stat within;
-                for (uint32_t b = 0; b < blocks_x; ++b)
-                    within.merge(partials[(static_cast<size_t>(s) * blocks_x + b) * comps + comp_offset + c]);
-                means.push_back(within.total() / static_cast<double>(e.units));
+                const BlockStat* row = partials + static_cast<size_t>(s) * blocks_x * comps + comp_offset + c;
+                for (uint32_t b = 0; b < blocks_x; ++b) {
+                    const BlockStat& bs = row[static_cast<size_t>(b) * comps];
+                    if (bs.n == 0) continue;
+                    within.add_sum(bs.sum);
+                    within.add_sum(bs.corr);
+                }
+                means[s] = within.total() / static_cast<double>(e.units);
             }
             stats[c] = Stat::two_pass(means);
         }
