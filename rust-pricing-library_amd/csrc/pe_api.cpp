@@ -1672,7 +1672,8 @@ struct LsmLaunch {
     std::vector<LsmItem> train_items, value_items;
     std::vector<uint32_t> first_partial, chunks, scrambles;
     DeviceBuffer d_trades, d_dates, d_sobol, d_train_items, d_value_items, store, cash, decisions, counts, in_sample,
-        partials, risk_partials, err, stops;
+        partials, risk_partials, err, stops, gram;
+    uint32_t gram_chunks = 0, max_dates = 0; // weighted-Laguerre pass partials per trade, most dates of a trade
     uint64_t n_trajectories = 0;             // valuation trajectories of all trades
     size_t n_decisions = 0, n_counts = 0, n_partials = 0;
     bool risk = false;                       // some plan asks for fixed-policy Greeks
@@ -1833,6 +1834,9 @@ LsmLaunch& prepare_lsm(std::vector<HostPlan*>& plans, pe_backend backend) {
             sobol.insert(sobol.end(), tab.begin(), tab.end());
         }
         L->trades.push_back(t);
+        L->gram_chunks = std::max<uint32_t>(
+            L->gram_chunks, static_cast<uint32_t>((t.train_paths + kLsmPassPaths - 1) / kLsmPassPaths));
+        L->max_dates = std::max(L->max_dates, n);
         auto add_items = [&](const LsmEngine& e, std::vector<LsmItem>& items, bool value) {
             const uint32_t ch = static_cast<uint32_t>((e.units + kItemUnits - 1) / kItemUnits);
             if (value) {
@@ -1882,6 +1886,10 @@ LsmLaunch& prepare_lsm(std::vector<HostPlan*>& plans, pe_backend backend) {
         L->host_risk_partials.resize(L->n_partials * kLsmRiskComponents);
     }
     L->err.reserve(sizeof(unsigned long long));
+    L->gram_chunks = std::max<uint32_t>(1, L->gram_chunks);
+    if (static_cast<uint64_t>(plans.size()) * L->gram_chunks > 0x7fffffffull)
+        throw PlanError(PE_ERR_SIZE_LIMIT, "too many training-path chunks");
+    L->gram.reserve(plans.size() * L->gram_chunks * 16 * sizeof(double));
     L->host_partials.resize(L->n_partials);
     L->host_decisions.resize(L->n_decisions);
     L->host_counts.resize(L->n_counts);
@@ -1930,6 +1938,9 @@ void evaluate_lsm_dual(HostPlan* plan, uint32_t n2, uint32_t n3, uint64_t seed, 
     a.partials = L.partials.ptr;
     a.error = L.err.as<unsigned long long>();
     a.n_trades = 1;
+    a.gram = L.gram.as<double>();
+    a.gram_chunks = L.gram_chunks;
+    a.max_dates = L.max_dates;
     LsmArgs train = a;
     train.items = L.d_train_items.as<LsmItem>();
     train.n_items = static_cast<uint32_t>(L.train_items.size());
@@ -1991,6 +2002,9 @@ void evaluate_lsm(std::vector<HostPlan*>& plans, pe_backend backend, pe_result* 
     a.risk_partials = L.risk ? L.risk_partials.ptr : nullptr;
     a.error = L.err.as<unsigned long long>();
     a.n_trades = static_cast<uint32_t>(plans.size());
+    a.gram = L.gram.as<double>();
+    a.gram_chunks = L.gram_chunks;
+    a.max_dates = L.max_dates;
     bool want_stops = false;
     for (size_t ti = 0; reports && ti < plans.size(); ++ti)
         want_stops = want_stops || (reports[ti].stopping_indices && reports[ti].stopping_capacity);

@@ -392,6 +392,8 @@ struct RoughNnArgs {
 
 // ------------------------------------------------------- early exercise --
 constexpr int kLsmThreads = 1024;     // induction workgroup: one per trade
+constexpr int kLsmPassThreads = 256;  // weighted-Laguerre induction pass: one workgroup per chunk
+constexpr uint32_t kLsmPassPaths = 8192;   // training paths per pass chunk
 constexpr int kLsmMaxBasis = 4;       // spot feature, degree <= 3
 constexpr int kLsmMaxDates = 1024;    // exercise dates per trade
 
@@ -486,6 +488,9 @@ struct LsmArgs {
     const LsmCuts* cuts;              // network policy: [trade][non-terminal date]
     uint32_t* stops;                  // optional: exercise-date index per valuation trajectory
     uint32_t n_items, n_trades;
+    double* gram;                     // weighted-Laguerre passes: 16 partial sums per (trade, chunk)
+    uint32_t gram_chunks;             // chunks per trade in `gram` (the largest trade's)
+    uint32_t max_dates;               // most exercise dates of any trade
 };
 
 // Andersen-Broadie bounds of one trained American trade (the valuation
