@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -788,24 +789,29 @@ double rough_scale(const HostPlan& p, double t) {
 }
 
 // The grid and model fields of RoughArgs: per-step dt, sqrt(dt) and the
-// right node's spot scale, the [step][factor] tables 1 + x_i dt and their
-// reciprocals, the lifted-Heston parameters.
+// right node's spot scale, the [step][slot] tables 1 + x_i dt and their
+// reciprocals, the lifted-Heston parameters.  Each step's row has the
+// kernel's factor slots (rough_factor_slots); the slots past the factors
+// hold +inf and 0, so those factors stay exactly zero and the kernels run
+// every slot without per-factor branches.
 void rough_grid_args(const HostPlan& p, const std::vector<double>& nodes, DeviceBuffer& d_steps, DeviceBuffer& d_den,
                      DeviceBuffer& d_rden, RoughArgs& a) {
     const HostLifted& h = p.lifted;
     const uint32_t n = static_cast<uint32_t>(nodes.size() - 1);
     const uint32_t nf = static_cast<uint32_t>(h.weights.size());
+    const uint32_t slots = rough_factor_slots(nf);
+    if (slots == 0) throw PlanError(PE_ERR_SIZE_LIMIT, "lifted Heston factor count");
     std::vector<RoughStep> steps(n);
-    std::vector<double> den(static_cast<size_t>(n) * nf);
-    std::vector<double> rden(den.size());
+    std::vector<double> den(static_cast<size_t>(n) * slots, std::numeric_limits<double>::infinity());
+    std::vector<double> rden(den.size(), 0.0);
     for (uint32_t j = 0; j < n; ++j) {
         const double dt = nodes[j + 1] - nodes[j];
         steps[j] = RoughStep{dt, std::sqrt(dt), rough_scale(p, nodes[j + 1]), 0.0};
         for (uint32_t i = 0; i < nf; ++i) {
             const double d = 1.0 + h.rates[i] * dt;
             if (!std::isfinite(d)) throw PlanError(PE_ERR_INVALID_ARGUMENT, "lifted time scale overflow");
-            den[static_cast<size_t>(j) * nf + i] = d;
-            rden[static_cast<size_t>(j) * nf + i] = 1.0 / d;
+            den[static_cast<size_t>(j) * slots + i] = d;
+            rden[static_cast<size_t>(j) * slots + i] = 1.0 / d;
         }
     }
     d_steps.upload(steps);

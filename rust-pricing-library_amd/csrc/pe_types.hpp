@@ -259,6 +259,13 @@ struct BsBatchArgs {
 // ------------------------------------------------------- lifted Heston --
 constexpr int kMaxRoughFactors = 32;
 
+// Factor slots of the kernel instantiation that runs `factors` factors (8,
+// 16, 20 or 32; 0 above kMaxRoughFactors): the [step][factor] tables are laid
+// out with this stride, padded with factors that stay at zero.
+__host__ __device__ constexpr uint32_t rough_factor_slots(uint32_t factors) {
+    return factors <= 8 ? 8u : factors <= 16 ? 16u : factors <= 20 ? 20u : factors <= 32 ? 32u : 0u;
+}
+
 struct RoughStep {       // per time step of the shared grid
     double dt, sqrt_dt;
     double scale_next;   // physical spot per unit of F at the step's right node
@@ -267,8 +274,8 @@ struct RoughStep {       // per time step of the shared grid
 
 struct RoughArgs {
     const RoughStep* steps;
-    const double* denominators;   // [step][factor]: 1 + x_i dt (FP64 mode divides by it)
-    const double* reciprocals;    // [step][factor]: 1 / (1 + x_i dt) (mixed mode multiplies)
+    const double* denominators;   // [step][slot]: 1 + x_i dt (FP64 mode divides by it), +inf past the factors
+    const double* reciprocals;    // [step][slot]: 1 / (1 + x_i dt) (mixed mode multiplies), 0 past the factors
     const uint32_t* sobol;        // [scramble][dim][33], dims = 2 * steps
     const uint32_t* store_nodes;  // ascending grid nodes whose spot is stored
     void* store;                  // [slot][path]: double (FP64 mode) or float
