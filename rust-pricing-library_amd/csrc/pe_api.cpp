@@ -1511,11 +1511,35 @@ LvLaunch& prepare_lv(std::vector<HostPlan*>& plans, pe_backend backend) {
     a.bin_cell = L.d_bins.as<uint16_t>();
     d_steps.upload(steps);
     d_node_weight.upload(node_weight);
+    a.n_ops = static_cast<uint32_t>(ops32.size());
+    a.ops_stride = 0;
+    if (e.kind == PE_ENGINE_RQMC && e.brownian_bridge) {
+        // one copy per scramble, each push carrying its dimension's direction
+        // words 0-5 of that scramble (the lane bits of the Sobol index)
+        const std::vector<uint32_t> tab = build_sobol_table(e.seed, e.scrambles, p0.sobol_dims);
+        const size_t n = ops32.size();
+        std::vector<LvOp32> copies(n * e.scrambles);
+        for (uint32_t sc = 0; sc < e.scrambles; ++sc)
+            for (size_t k = 0; k < n; ++k) {
+                LvOp32 o = ops32[k];
+                if ((o.code & 3u) == kLvPush) {
+                    const uint32_t dim = static_cast<uint32_t>(o.index);
+                    if (dim >= p0.sobol_dims) throw PlanError(PE_ERR_INVALID_ARGUMENT, "bridge dimension out of range");
+                    uint32_t words[12];
+                    std::memcpy(words, &o, sizeof(o));
+                    for (int b = 0; b < 6; ++b)
+                        words[kLvPushDirWord + b] = tab[(static_cast<size_t>(sc) * p0.sobol_dims + dim) * 33u + b];
+                    std::memcpy(&o, words, sizeof(o));
+                }
+                copies[static_cast<size_t>(sc) * n + k] = o;
+            }
+        ops32.swap(copies);
+        a.ops_stride = static_cast<uint32_t>(n);
+    }
     d_ops32.upload(ops32);
     d_ops64.upload(ops64);
     a.ops32 = d_ops32.as<LvOp32>();
     a.ops64 = d_ops64.as<LvOp64>();
-    a.n_ops = static_cast<uint32_t>(ops32.size());
     a.steps = d_steps.as<LvStep>();
     a.program = p0.d_program.as<BridgeOp>();
     a.grid = p0.d_grid.as<double>();

@@ -3,6 +3,7 @@
 // copied to HBM as a table.
 #pragma once
 
+#include <stddef.h>
 #include <stdint.h>
 
 namespace pe {
@@ -175,6 +176,14 @@ struct LvOp32 {          // mixed kernel (12 dwords)
     float m_half_dt, sqrt_dt, t_weight, next_forward;
     float pad0, pad1;
 };
+// An RQMC bridge program has one copy of the stream per scramble; in each
+// copy a push's dwords 6-11 (step fields, unused by a push) hold that
+// scramble's direction words 0-5 of the push's Sobol dimension, so the
+// push's record brings the lane part of its Sobol word in the same load.
+constexpr int kLvPushDirWord = 6;
+static_assert(offsetof(LvOp32, m_half_dt) == kLvPushDirWord * 4 && offsetof(LvOp32, pad1) == (kLvPushDirWord + 5) * 4 &&
+                  sizeof(LvOp32) == 48,
+              "a push's direction words fill dwords 6-11");
 
 struct LvOp64 {          // FP64 kernel (the reference's constants, bit for bit)
     uint32_t code;
@@ -548,7 +557,7 @@ struct LvArgs {
     const LvOp32* ops32;       // fused program (bridge pushes + steps), mixed kernel
     const LvOp64* ops64;       // same program with FP64 constants
     uint32_t n_ops;
-    uint32_t pad_ops;
+    uint32_t ops_stride;       // records per scramble copy of ops32 (0: one shared copy)
     ProductHot hot[kMaxProducts];
     // Uniform bins over [x0, xl] (mixed kernel): the FP32 coordinate xx maps
     // to bin trunc(fma(xx, inv_bin_w32, bin_off32)); bin_cell[b] counts the
