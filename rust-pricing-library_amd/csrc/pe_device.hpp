@@ -40,6 +40,17 @@ PE_HD U4 philox_block(uint64_t seed, uint64_t path, uint32_t quad, uint32_t doma
     return philox4x32_10(seed, U4{static_cast<uint32_t>(path), static_cast<uint32_t>(path >> 32), quad, domain});
 }
 
+// The same block for a wave-uniform seed, with the ten rounds' keys formed
+// here on the scalar unit: the seed passes through an empty asm statement,
+// so the key schedule is not hoisted out of the caller's loops into twenty
+// scalar registers that spill into vector lanes.  Bit-identical words.
+__device__ __forceinline__ U4 philox_block_uniform(uint64_t seed, uint64_t path, uint32_t quad, uint32_t domain) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+s"(seed));
+#endif
+    return philox_block(seed, path, quad, domain);
+}
+
 PE_HD uint32_t lane_of(const U4& b, uint32_t lane) {
     return lane == 0 ? b.x : lane == 1 ? b.y : lane == 2 ? b.z : b.w;
 }
